@@ -1,0 +1,207 @@
+/*
+ * zodi_hip.h — C ABI of the MI355X (gfx950) zodiacal line-of-sight integral.
+ *
+ * Plain C types only: device pointers, sizes and an opaque HIP stream handle.
+ * Every entry point returns a zodi_status (0 on success). All buffers are owned
+ * by the caller and live in device memory unless stated otherwise; calls are
+ * asynchronous on `stream` (NULL = the default stream).
+ *
+ * Reference interfaces replaced (paths relative to the zodipy 1.1.3 tree):
+ *   zodi_forward          zodipy/model.py:253-277   (get_line_of_sight_range +
+ *                         the components x Gauss-Legendre nodes loop of
+ *                         Model._evaluate, i.e. zodipy/line_of_sight.py:55-105,
+ *                         zodipy/brightness.py:21-83, zodipy/number_density.py:47-404,
+ *                         zodipy/scattering.py:11-59, zodipy/blackbody.py:16-30)
+ *   zodi_line_of_sight    zodipy/line_of_sight.py:88-105 (per-component start/stop)
+ *   zodi_number_density   zodipy/number_density.py:47-420 (component densities at
+ *                         heliocentric points; the kernel behind grid_number_density,
+ *                         zodipy/number_density.py:482-536)
+ *   zodi_adjoint          no reference counterpart: vector-Jacobian product of the
+ *                         forward map with respect to the model parameter vector
+ *                         (the order of zodipy/zodiacal_light_model.py:38-55's
+ *                         parameter dictionary, followed by the effective spectral
+ *                         parameters), for parameter fitting
+ */
+#ifndef ZODI_HIP_H
+#define ZODI_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ZODI_ABI_VERSION 1
+
+#define ZODI_MAX_COMPS 16
+#define ZODI_MAX_GROUPS 16
+#define ZODI_MAX_NODES 512
+#define ZODI_N_TEMPS 100
+#define ZODI_COMP_SLOTS 24
+#define ZODI_SHARED_SLOTS 8
+
+typedef int32_t zodi_status;
+enum {
+  ZODI_OK = 0,
+  ZODI_ERR_ARG = 1,        /* invalid argument (null pointer, size, mode) */
+  ZODI_ERR_MODEL = 2,      /* malformed model block (version, counts) */
+  ZODI_ERR_HIP = 3,        /* a HIP runtime call failed; see zodi_last_error() */
+  ZODI_ERR_TOO_LARGE = 4   /* size beyond a documented limit */
+};
+
+/* Integrand families (zodipy/brightness.py:21-56 and :59-83). */
+enum { ZODI_INTEGRAND_KELSALL = 0, ZODI_INTEGRAND_RRM = 1 };
+
+/* Component kinds (zodipy/number_density.py:408-420). */
+enum {
+  ZODI_CLOUD = 0,
+  ZODI_BAND = 1,
+  ZODI_RING = 2,
+  ZODI_FEATURE = 3,
+  ZODI_FAN = 4,
+  ZODI_COMET = 5,
+  ZODI_INTERSTELLAR = 6,
+  ZODI_NARROW_BAND = 7,
+  ZODI_BROAD_BAND = 8,
+  ZODI_RING_RRM = 9,
+  ZODI_FEATURE_RRM = 10
+};
+
+/* comp_flags bits */
+#define ZODI_CF_CENTERED 1u   /* x_0 = y_0 = z_0 = 0: the component frame is heliocentric */
+#define ZODI_CF_SCATTER 2u    /* albedo != 0: add scattered sunlight (zodipy/brightness.py:50) */
+#define ZODI_CF_P_IS_4 4u     /* band exponent p == 4 exactly */
+
+/* Per-component parameter slots, common part. */
+enum { ZODI_P_X0 = 0, ZODI_P_Y0 = 1, ZODI_P_Z0 = 2,
+       ZODI_P_NX = 3,   /* sin(Omega) sin(i) */
+       ZODI_P_NY = 4,   /* cos(Omega) sin(i)  (enters with a minus sign) */
+       ZODI_P_NZ = 5,   /* cos(i) */
+       ZODI_P_K0 = 6,   /* first kind-specific slot */
+       ZODI_P_EMISSIVITY = 20, ZODI_P_ALBEDO = 21, ZODI_P_ONE_MINUS_ALBEDO = 22 };
+
+/* Shared parameter slots. */
+enum { ZODI_S_C1 = 0, ZODI_S_C2 = 1, ZODI_S_C3 = 2, ZODI_S_SOLAR = 3,
+       ZODI_S_PHASE_NORM = 4, ZODI_S_CALIBRATION = 5 };
+
+/*
+ * Device model block: uploaded once per model construction or parameter update.
+ * Components are listed in model order and partitioned into "groups": maximal
+ * runs of consecutive components that share the line-of-sight range (cutoff
+ * radii) and the dust temperature law, so position, temperature and Planck
+ * lookup are evaluated once per group and node.
+ *
+ * Kind-specific slots, from ZODI_P_K0:
+ *   CLOUD        n_0, alpha, beta, gamma, mu
+ *   BAND         3*n_0, delta_zeta_rad, p, v, delta_r
+ *   RING         n_0, R, sigma_r, sigma_z
+ *   FEATURE      n_0, R, sigma_r, sigma_z, theta_rad, sigma_theta_rad
+ *   FAN          Q, P, gamma, Z_0, R_outer
+ *   COMET        gamma, Z_0, P, amp, R_inner, R_outer
+ *   INTERSTELLAR amp
+ *   NARROW_BAND  beta_nb, G, gamma, A, R_inner, R_outer
+ *   BROAD_BAND   beta_bb, sigma_bb, gamma, A, R_inner, R_outer
+ *   RING_RRM     n_0, R, sigma_r, sigma_z, A
+ *   FEATURE_RRM  n_0, R, sigma_r, sigma_z, theta_rad, sigma_theta_rad, A
+ */
+typedef struct zodi_model {
+  int32_t abi_version;
+  int32_t integrand;
+  int32_t n_comps;
+  int32_t n_groups;
+  int32_t n_nodes;
+  int32_t n_temps;
+  int32_t flags;
+  int32_t reserved;
+  int32_t comp_kind[ZODI_MAX_COMPS];
+  uint32_t comp_flags[ZODI_MAX_COMPS];
+  int32_t group_first[ZODI_MAX_GROUPS];
+  int32_t group_count[ZODI_MAX_GROUPS];
+  double group_r_in[ZODI_MAX_GROUPS];   /* start cutoff radius [AU] */
+  double group_r_out[ZODI_MAX_GROUPS];  /* stop cutoff radius [AU] */
+  double group_T0[ZODI_MAX_GROUPS];     /* T(R) = T0 R^-delta */
+  double group_delta[ZODI_MAX_GROUPS];
+  double shared[ZODI_SHARED_SLOTS];
+  double comp[ZODI_MAX_COMPS][ZODI_COMP_SLOTS];
+  double gl_x[ZODI_MAX_NODES];          /* Gauss-Legendre nodes on [-1, 1], ascending */
+  double gl_w[ZODI_MAX_NODES];
+  double tab_T[ZODI_N_TEMPS];           /* Planck table temperatures [K] */
+  double tab_B[ZODI_N_TEMPS];           /* B_nu(T) [MJy/sr], delta or bandpass-averaged */
+  double tab_slope[ZODI_N_TEMPS];       /* (B[j+1]-B[j])/(T[j+1]-T[j]) */
+  double tab_inv_step;                  /* (n_temps-1)/(T[last]-T[0]) for the index guess */
+  double pad_[7];
+} zodi_model;
+
+/* Observer addressing modes. */
+enum {
+  ZODI_OBS_PER_SEGMENT = 0, /* obs/earth are [3][n_seg]; ray r belongs to segment s with
+                               seg_offsets[s] <= r < seg_offsets[s+1] */
+  ZODI_OBS_PER_RAY = 1      /* obs/earth are [3][n_rays] (time-ordered data) */
+};
+
+/* Output modes. */
+enum {
+  ZODI_OUT_COMPS = 0,  /* out is [n_comps][n_rays] */
+  ZODI_OUT_SUM = 1     /* out is [n_rays], components summed in model order */
+};
+
+/* Launch configuration (0 in any field selects the built-in default). */
+typedef struct zodi_launch {
+  int32_t block;          /* threads per block: 64, 128, 256, 512 */
+  int32_t lanes_per_ray;  /* 1, 2, 4, 8: Gauss-Legendre nodes split over this many lanes */
+  int32_t blocks_per_cu;  /* persistent grid: resident blocks per CU; 0 = one block per tile */
+  int32_t variant;        /* 0 generic, 1 specialised layout if available */
+} zodi_launch;
+
+/* Library identity. */
+int32_t zodi_abi_version(void);
+int64_t zodi_model_size(void);             /* sizeof(zodi_model) */
+const char* zodi_last_error(void);         /* thread-local message of the last failure */
+
+/*
+ * Every compute entry point takes the model block twice: `model_host` (host
+ * memory) is validated and drives launch decisions; `model_dev` (device memory)
+ * must hold the same bytes and is what the kernels read.
+ *
+ * zodi_forward — integrated emission per ray [MJy/sr].
+ *
+ * u          [3][n_rays] ecliptic unit vectors (SoA rows x, y, z)
+ * seg_offsets[n_seg+1] ray offsets per observer (PER_SEGMENT), ignored for PER_RAY
+ * obs, earth [3][n_obs] heliocentric ecliptic positions [AU]; n_obs = n_seg or n_rays
+ * beyond     [n_seg] (PER_SEGMENT) or [1] (PER_RAY): bit 2g set when the observer lies
+ *            outside group g's start sphere, bit 2g+1 for its stop sphere; such a range
+ *            end is float64 epsilon (zodipy/line_of_sight.py:72-73: the test covers every
+ *            observer of one reference call)
+ * out        see ZODI_OUT_*
+ */
+zodi_status zodi_forward(const zodi_model* model_host, const zodi_model* model_dev,
+                         const double* u, int64_t n_rays,
+                         const int64_t* seg_offsets, int32_t n_seg, const double* obs,
+                         const double* earth, int32_t obs_mode, const uint32_t* beyond,
+                         double* out, int32_t out_mode, const zodi_launch* launch,
+                         void* stream);
+
+/*
+ * zodi_line_of_sight — per-group start and stop distances [AU]: start/stop are
+ * [n_groups][n_rays]. Same addressing as zodi_forward.
+ */
+zodi_status zodi_line_of_sight(const zodi_model* model_host, const zodi_model* model_dev,
+                               const double* u, int64_t n_rays,
+                               const int64_t* seg_offsets, int32_t n_seg, const double* obs,
+                               int32_t obs_mode, const uint32_t* beyond, double* start,
+                               double* stop, void* stream);
+
+/*
+ * zodi_number_density — component densities at heliocentric points:
+ * X [3][n_points] -> out [n_comps][n_points]; earth [3] is the Earth position
+ * used by the feature components.
+ */
+zodi_status zodi_number_density(const zodi_model* model_host, const zodi_model* model_dev,
+                                const double* X, int64_t n_points,
+                                const double* earth, double* out, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ZODI_HIP_H */

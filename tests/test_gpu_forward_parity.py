@@ -1,0 +1,240 @@
+"""GPU parity of the fused forward kernel against the CPU oracle.
+
+Tolerance: 1e-10 relative per element (NaN must match NaN, exact zeros must be
+exact zeros). Inputs are seeded; sizes keep the oracle to a few seconds.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+import torch
+
+from oracle.zodi_oracle import OracleModel, evaluate_segments
+from zodipy_amd import Model, healpix
+from zodipy_amd import units as u
+from zodipy_amd._lib import Launch
+from zodipy_amd.runtime import OUT_COMPS, OUT_SUM
+
+pytestmark = pytest.mark.gpu
+
+RTOL = 1e-10
+
+
+def assert_parity(got, want, rtol=RTOL):
+    got = np.asarray(got)
+    want = np.asarray(want)
+    assert got.shape == want.shape
+    nan_g, nan_w = np.isnan(got), np.isnan(want)
+    assert np.array_equal(nan_g, nan_w), f"NaN mismatch: {nan_g.sum()} vs {nan_w.sum()}"
+    g, w = got[~nan_w], want[~nan_w]
+    zero = w == 0
+    assert np.all(g[zero] == 0), "exact zeros differ"
+    rel = np.abs(g - w) / np.maximum(np.abs(w), np.finfo(float).tiny)
+    worst = rel.max(initial=0.0)
+    assert worst <= rtol, f"max relative error {worst:.3e} > {rtol:.1e}"
+
+
+def random_dirs(n, seed):
+    rng = np.random.default_rng(seed)
+    z = rng.uniform(-1, 1, n)
+    phi = rng.uniform(0, 2 * np.pi, n)
+    s = np.sqrt(1 - z * z)
+    return np.stack([s * np.cos(phi), s * np.sin(phi), z])
+
+
+def test_config1_dirbe_25um_nside16(gpu):
+    """DIRBE, 25 um, nside=16 pixel centres, observer = Earth = (1, 0, 0) AU."""
+    v = healpix.pix2vec(16)
+    obs = np.array([1.0, 0.0, 0.0])
+    model = Model(25 * u.micron)
+    got = model.evaluate_vectors(v, obspos=obs, earthpos=obs, return_comps=True).value
+    want = OracleModel(25 * u.micron).evaluate_vectors(v, obs, obs)
+    assert_parity(got, want)
+
+
+@pytest.mark.parametrize(
+    "name,x,kw",
+    [
+        ("dirbe", 1.25 * u.micron, {}),  # scattering branch (albedo != 0)
+        ("dirbe", 3.5 * u.micron, {}),
+        ("dirbe", 140 * u.micron, {}),
+        ("planck13", 857 * u.GHz, {}),
+        ("planck15", 353 * u.GHz, {}),
+        ("planck18", 545 * u.GHz, {}),
+        ("odegard", 100 * u.GHz, {}),
+        ("rrm-experimental", 25 * u.micron, {}),
+        ("rrm-experimental", 75 * u.micron, {}),
+        ("dirbe", 0.9 * u.micron, {"extrapolate": True}),
+    ],
+)
+def test_registry_models_off_ecliptic_observer(gpu, name, x, kw):
+    """Every registered model; observer off the ecliptic (z0 enters c but not b)."""
+    v = random_dirs(2000, seed=7)
+    obs = np.array([0.62, -0.78, 0.05])
+    earth = np.array([0.60, -0.80, 0.0])
+    model = Model(x, name=name, **kw)
+    got = model.evaluate_vectors(v, obspos=obs, earthpos=earth, return_comps=True).value
+    want = OracleModel(x, name=name, **kw).evaluate_vectors(v, obs, earth)
+    assert_parity(got, want)
+
+
+def test_bandpass_gaussian_25um(gpu):
+    """Config-2 style bandpass in microns (normalised Gaussian weights)."""
+    x = np.linspace(17.5, 32.5, 50) * u.micron
+    w = np.exp(-0.5 * ((x.value - 25) / 2.5) ** 2)
+    v = healpix.pix2vec(8)
+    obs = np.array([np.cos(0.3), np.sin(0.3), 0.0])
+    got = Model(x, weights=w).evaluate_vectors(v, obspos=obs, earthpos=obs,
+                                               return_comps=True).value
+    want = OracleModel(x, weights=w).evaluate_vectors(v, obs, obs)
+    assert_parity(got, want)
+
+
+def test_bandpass_ghz_on_micron_model(gpu):
+    """Bandpass in GHz on the micron-tabulated DIRBE model (unit-axis quirk kept)."""
+    x = np.linspace(11000, 13000, 20) * u.GHz
+    w = np.ones(20)
+    v = random_dirs(500, seed=3)
+    obs = np.array([1.0, 0.0, 0.0])
+    got = Model(x, weights=w).evaluate_vectors(v, obspos=obs, earthpos=obs,
+                                               return_comps=True).value
+    want = OracleModel(x, weights=w).evaluate_vectors(v, obs, obs)
+    assert_parity(got, want)
+
+
+def test_time_ordered_per_ray_observers(gpu):
+    """Per-ray observers: one cutoff test covers all observers of the call."""
+    n = 1500
+    v = random_dirs(n, seed=11)
+    phase = np.linspace(0, 1.0, n)
+    obs = np.stack([np.cos(phase), np.sin(phase), 0.01 * np.sin(3 * phase)])
+    earth = np.stack([np.cos(phase), np.sin(phase), np.zeros(n)])
+    model = Model(12 * u.micron)
+    got = model.evaluate_vectors(v, obspos=obs, earthpos=earth, return_comps=True).value
+    want = OracleModel(12 * u.micron).evaluate_vectors(v, obs, earth)
+    assert_parity(got, want)
+    # one observer beyond the ring/feature cutoffs zeroes those components for every ray
+    obs2 = obs.copy()
+    obs2[:, 17] *= 1.35
+    got2 = model.evaluate_vectors(v, obspos=obs2, earthpos=earth, return_comps=True).value
+    want2 = OracleModel(12 * u.micron).evaluate_vectors(v, obs2, earth)
+    assert_parity(got2, want2)
+    assert np.all(want2[4:] == 0)
+
+
+def test_batched_segments_uneven(gpu):
+    """Many observers with uneven (including empty) pointing counts in one launch."""
+    rng = np.random.default_rng(5)
+    counts = rng.integers(0, 700, 23)
+    counts[[3, 10]] = 0
+    n = int(counts.sum())
+    seg = np.concatenate([[0], np.cumsum(counts)]).astype(np.int64)
+    v = random_dirs(n, seed=9)
+    ph = rng.uniform(0, 2 * np.pi, counts.size)
+    obs = np.stack([np.cos(ph), np.sin(ph), np.zeros_like(ph)])
+    obs[:, 4] *= 1.4  # this observer is outside the ring/feature spheres
+    model = Model(12 * u.micron)
+    dm = model.device_model
+    got = dm.forward(v, obs, obs, seg_offsets=seg, out_mode=OUT_COMPS).cpu().numpy()
+    want = evaluate_segments(OracleModel(12 * u.micron), v, seg, obs, obs)
+    assert_parity(got, want)
+    tot = dm.forward(v, obs, obs, seg_offsets=seg, out_mode=OUT_SUM).cpu().numpy()
+    assert np.array_equal(tot, got.sum(axis=0))
+
+
+@pytest.mark.parametrize("block,lanes,per_cu", [(64, 1, 0), (128, 2, 0), (256, 4, 2),
+                                                (512, 8, 1), (256, 1, 4)])
+def test_launch_variants_agree(gpu, block, lanes, per_cu):
+    rng = np.random.default_rng(2)
+    counts = rng.integers(1, 400, 9)
+    seg = np.concatenate([[0], np.cumsum(counts)]).astype(np.int64)
+    v = random_dirs(int(counts.sum()), seed=21)
+    ph = rng.uniform(0, 2 * np.pi, counts.size)
+    obs = np.stack([np.cos(ph), np.sin(ph), np.zeros_like(ph)])
+    model = Model(25 * u.micron)
+    dm = model.device_model
+    launch = Launch(block=block, lanes_per_ray=lanes, blocks_per_cu=per_cu, variant=0)
+    got = dm.forward(v, obs, obs, seg_offsets=seg, launch=launch).cpu().numpy()
+    want = evaluate_segments(OracleModel(25 * u.micron), v, seg, obs, obs)
+    assert_parity(got, want)
+
+
+def test_return_comps_sum_is_exact(gpu):
+    v = random_dirs(777, seed=1)
+    obs = np.array([0.3, 0.95, 0.0])
+    model = Model(20 * u.micron)
+    comps = model.evaluate_vectors(v, obspos=obs, earthpos=obs, return_comps=True)
+    total = model.evaluate_vectors(v, obspos=obs, earthpos=obs, return_comps=False)
+    assert comps.shape == (6, 777) and total.shape == (777,)
+    assert np.array_equal(comps.value.sum(axis=0), total.value)
+
+
+@pytest.mark.parametrize("degree", [1, 7, 100])
+def test_gauss_quad_degree(gpu, degree):
+    v = random_dirs(300, seed=4)
+    obs = np.array([1.0, 0.0, 0.0])
+    got = Model(25 * u.micron, gauss_quad_degree=degree).evaluate_vectors(
+        v, obspos=obs, earthpos=obs, return_comps=True).value
+    want = OracleModel(25 * u.micron, gauss_quad_degree=degree).evaluate_vectors(v, obs, obs)
+    assert_parity(got, want)
+
+
+def test_special_directions_and_nan(gpu):
+    """Poles, axis directions, and b == 0 exactly (observer on the z axis) -> NaN."""
+    v = np.array([[0, 0, 1], [0, 0, -1], [1, 0, 0], [-1, 0, 0], [0, 1, 0], [0, -1, 0]],
+                 dtype=np.float64).T
+    for obs in (np.array([1.0, 0.0, 0.0]), np.array([0.0, 0.0, 0.5]),
+                np.array([0.0, 1.0, 0.2])):
+        got = Model(25 * u.micron).evaluate_vectors(v, obspos=obs, earthpos=obs,
+                                                    return_comps=True).value
+        want = OracleModel(25 * u.micron).evaluate_vectors(v, obs, obs)
+        assert_parity(got, want)
+
+
+def test_line_of_sight_kernel(gpu):
+    from oracle.zodi_oracle import get_sphere_intersection
+    from zodipy_amd.cutoffs import COMPONENT_CUTOFFS
+
+    v = random_dirs(1000, seed=8)
+    obs = np.array([0.9, 0.3, 0.02])
+    for name in ("dirbe", "rrm-experimental"):
+        model = Model(25 * u.micron, name=name)
+        dm = model.device_model
+        start, stop = dm.line_of_sight(v, obs)
+        start, stop = start.cpu().numpy(), stop.cpu().numpy()
+        for c, label in enumerate(model.comps):
+            g = dm.packed.group_of_comp[c]
+            r_in, r_out = COMPONENT_CUTOFFS[label]
+            want0 = np.broadcast_to(get_sphere_intersection(obs[:, None], v, r_in), (1000,))
+            want1 = np.broadcast_to(get_sphere_intersection(obs[:, None], v, r_out), (1000,))
+            assert_parity(start[g], want0, rtol=1e-14)
+            assert_parity(stop[g], want1, rtol=1e-14)
+
+
+def test_number_density_kernel(gpu):
+    from zodipy_amd.number_density import density_device_model
+    from zodipy_amd.model_registry import model_registry
+
+    rng = np.random.default_rng(12)
+    X = rng.uniform(-3.5, 3.5, (3, 4000))
+    X[2] *= 0.3
+    earth = np.array([0.7, -0.7, 0.0])
+    for name in ("dirbe", "rrm-experimental"):
+        zm = model_registry.get_model(name)
+        got = density_device_model(zm).number_density(X, earth).cpu().numpy()
+        want = OracleModel(25 * u.micron, name=name).number_density(X, earth)
+        assert_parity(got, want)
+
+
+def test_device_tensor_output_and_stream(gpu):
+    v = torch.as_tensor(random_dirs(4096, seed=30), device="cuda")
+    obs = np.array([1.0, 0.0, 0.0])
+    model = Model(25 * u.micron)
+    s = torch.cuda.Stream()
+    with torch.cuda.stream(s):
+        out = model.evaluate_vectors(v, obspos=obs, earthpos=obs, as_tensor=True)
+    s.synchronize()
+    assert out.is_cuda and out.shape == (4096,)
+    want = OracleModel(25 * u.micron).evaluate_vectors(v.cpu().numpy(), obs, obs).sum(axis=0)
+    assert_parity(out.cpu().numpy(), want)

@@ -1,0 +1,475 @@
+// Fused FP64 line-of-sight kernels for MI355X (gfx950) and their C ABI.
+//
+// One ray per lane (or per `L` adjacent lanes, which then split the
+// Gauss-Legendre nodes and combine with cross-lane shuffles). For each ray the
+// per-ray sphere-intersection setup runs once; then, per group of components
+// sharing a line-of-sight range and temperature law, the node loop evaluates
+// position, temperature and the Planck lookup once and every component density
+// of the group on top, accumulating in registers. Parameters are read with
+// wave-uniform addresses (scalar loads); the Planck table is staged in LDS.
+//
+// Reference: zodipy/model.py:253-277 and the functions it calls (see zodi_hip.h).
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+#include "zodi_device.hpp"
+#include "zodi_hip.h"
+
+namespace zodi {
+
+namespace {
+thread_local std::string g_last_error;
+
+zodi_status fail(zodi_status code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
+zodi_status fail(zodi_status code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_last_error = buf;
+  return code;
+}
+
+zodi_status check_hip(hipError_t e, const char* what) {
+  if (e != hipSuccess) {
+    return fail(ZODI_ERR_HIP, "%s: %s", what, hipGetErrorString(e));
+  }
+  return ZODI_OK;
+}
+
+int cu_count() {
+  static int counts[64] = {0};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+  if (counts[dev] == 0) {
+    int n = 0;
+    if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        n <= 0) {
+      n = 256;
+    }
+    counts[dev] = n;
+  }
+  return counts[dev];
+}
+
+zodi_status validate_model(const zodi_model* m) {
+  if (m == nullptr) return fail(ZODI_ERR_ARG, "model_host is null");
+  if (m->abi_version != ZODI_ABI_VERSION)
+    return fail(ZODI_ERR_MODEL, "model abi_version %d, library %d", m->abi_version,
+                ZODI_ABI_VERSION);
+  if (m->integrand != ZODI_INTEGRAND_KELSALL && m->integrand != ZODI_INTEGRAND_RRM)
+    return fail(ZODI_ERR_MODEL, "unknown integrand %d", m->integrand);
+  if (m->n_comps < 1 || m->n_comps > ZODI_MAX_COMPS)
+    return fail(ZODI_ERR_MODEL, "n_comps %d outside [1, %d]", m->n_comps, ZODI_MAX_COMPS);
+  if (m->n_groups < 1 || m->n_groups > ZODI_MAX_GROUPS)
+    return fail(ZODI_ERR_MODEL, "n_groups %d outside [1, %d]", m->n_groups, ZODI_MAX_GROUPS);
+  if (m->n_nodes < 1 || m->n_nodes > ZODI_MAX_NODES)
+    return fail(ZODI_ERR_MODEL, "n_nodes %d outside [1, %d]", m->n_nodes, ZODI_MAX_NODES);
+  if (m->n_temps < 2 || m->n_temps > ZODI_N_TEMPS)
+    return fail(ZODI_ERR_MODEL, "n_temps %d outside [2, %d]", m->n_temps, ZODI_N_TEMPS);
+  int next = 0;
+  for (int g = 0; g < m->n_groups; ++g) {
+    if (m->group_first[g] != next)
+      return fail(ZODI_ERR_MODEL, "group %d starts at %d, expected %d", g, m->group_first[g],
+                  next);
+    if (m->group_count[g] < 1 || m->group_count[g] > kMaxSlots)
+      return fail(ZODI_ERR_MODEL, "group %d has %d components (1..%d)", g, m->group_count[g],
+                  kMaxSlots);
+    next += m->group_count[g];
+  }
+  if (next != m->n_comps)
+    return fail(ZODI_ERR_MODEL, "groups cover %d components, model has %d", next, m->n_comps);
+  for (int c = 0; c < m->n_comps; ++c) {
+    if (m->comp_kind[c] < ZODI_CLOUD || m->comp_kind[c] > ZODI_FEATURE_RRM)
+      return fail(ZODI_ERR_MODEL, "component %d has unknown kind %d", c, m->comp_kind[c]);
+  }
+  return ZODI_OK;
+}
+
+}  // namespace
+
+// Kernel arguments (passed by value in the kernarg segment).
+struct FwdArgs {
+  const zodi_model* m;
+  const double* u;
+  int64_t n_rays;
+  const int64_t* seg;
+  int32_t n_seg;
+  int32_t obs_mode;
+  const double* obs;
+  const double* earth;
+  int64_t obs_stride;  // row length of obs/earth: n_seg or n_rays
+  const uint32_t* beyond;
+  double* out;
+  int32_t out_mode;
+  int32_t pad;
+  int64_t n_tiles;
+};
+
+// Index of the observer of ray r (largest s with seg[s] <= r); `s0` is that
+// index for an earlier ray of the same tile.
+__device__ __forceinline__ int64_t walk_segment(const int64_t* __restrict__ seg, int32_t n_seg,
+                                                int64_t s0, int64_t r) {
+  int64_t s = s0;
+  while (s + 1 < n_seg && seg[s + 1] <= r) ++s;
+  return s;
+}
+
+__device__ __forceinline__ int64_t find_segment(const int64_t* __restrict__ seg, int32_t n_seg,
+                                                int64_t r) {
+  int32_t lo = 0, hi = n_seg - 1;
+  while (lo < hi) {
+    const int32_t mid = (lo + hi + 1) >> 1;
+    if (seg[mid] <= r) {
+      lo = mid;
+    } else {
+      hi = mid - 1;
+    }
+  }
+  return lo;
+}
+
+__device__ __forceinline__ void load_table(const zodi_model* __restrict__ M, double* sT,
+                                           double* sB, double* sS) {
+  const int n = M->n_temps;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    sT[i] = M->tab_T[i];
+    sB[i] = M->tab_B[i];
+    sS[i] = M->tab_slope[i];
+  }
+  __syncthreads();
+}
+
+// Generic fused forward kernel: any component layout allowed by the model block.
+template <int BLOCK, int L>
+__global__ __launch_bounds__(BLOCK) void forward_generic(FwdArgs a) {
+  __shared__ double sT[ZODI_N_TEMPS];
+  __shared__ double sB[ZODI_N_TEMPS];
+  __shared__ double sS[ZODI_N_TEMPS];
+  const zodi_model* __restrict__ M = a.m;
+  load_table(M, sT, sB, sS);
+
+  const double* __restrict__ U = a.u;
+  const double* __restrict__ OBS = a.obs;
+  const double* __restrict__ EARTH = a.earth;
+  const int64_t* __restrict__ SEG = a.seg;
+  double* __restrict__ OUT = a.out;
+
+  const int n_t = M->n_temps;
+  const double inv_step = M->tab_inv_step;
+  const int n_groups = M->n_groups;
+  const int K = M->n_nodes;
+  const int integrand = M->integrand;
+  const double* __restrict__ S = M->shared;
+  const double calibration = S[ZODI_S_CALIBRATION];
+  const double solar = S[ZODI_S_SOLAR];
+
+  constexpr int RAYS = BLOCK / L;
+  const int sub = threadIdx.x % L;
+  const int local = threadIdx.x / L;
+  const int64_t n = a.n_rays;
+
+  for (int64_t tile = blockIdx.x; tile < a.n_tiles; tile += gridDim.x) {
+    const int64_t r = tile * RAYS + local;
+    const bool active = r < n;
+    const int64_t rr = active ? r : n - 1;
+
+    int64_t s;
+    if (a.obs_mode == ZODI_OBS_PER_RAY) {
+      s = rr;
+    } else {
+      const int64_t s0 = find_segment(SEG, a.n_seg, tile * RAYS);
+      s = walk_segment(SEG, a.n_seg, s0, rr);
+    }
+    const uint32_t bits = a.beyond[a.obs_mode == ZODI_OBS_PER_RAY ? 0 : s];
+
+    const double ux = U[rr], uy = U[n + rr], uz = U[2 * n + rr];
+    const int64_t os = a.obs_stride;
+    const double ox = OBS[s], oy = OBS[os + s], oz = OBS[2 * os + s];
+    const double ex = EARTH[s], ey = EARTH[os + s];
+    const RayGeom G = ray_geometry(ux, uy, uz, ox, oy, oz);
+
+    double total = 0.0;
+    for (int g = 0; g < n_groups; ++g) {
+      const int first = M->group_first[g];
+      const int cnt = M->group_count[g];
+      const double start = ((bits >> (2 * g)) & 1u) ? kEps : sphere_distance(G, M->group_r_in[g]);
+      const double stop =
+          ((bits >> (2 * g + 1)) & 1u) ? kEps : sphere_distance(G, M->group_r_out[g]);
+      const double h = 0.5 * (stop - start);
+      const double mid = 0.5 * (stop + start);
+      const double T0 = M->group_T0[g];
+      const double mdelta = -M->group_delta[g];
+
+      double elon[kMaxSlots];
+      bool scatter = false;
+#pragma unroll
+      for (int j = 0; j < kMaxSlots; ++j) {
+        elon[j] = 0.0;
+        if (j < cnt) {
+          const int c = first + j;
+          const double* __restrict__ P = M->comp[c];
+          if (is_feature(M->comp_kind[c])) elon[j] = atan2(ey - P[ZODI_P_Y0], ex - P[ZODI_P_X0]);
+          scatter = scatter || (M->comp_flags[c] & ZODI_CF_SCATTER);
+        }
+      }
+
+      double acc[kMaxSlots];
+#pragma unroll
+      for (int j = 0; j < kMaxSlots; ++j) acc[j] = 0.0;
+
+      for (int k = sub; k < K; k += L) {
+        const double xk = M->gl_x[k];
+        const double wk = M->gl_w[k];
+        const double s_los = h * xk + mid;
+        const double X0 = s_los * ux + ox;
+        const double X1 = s_los * uy + oy;
+        const double X2 = s_los * uz + oz;
+        const double R = sqrt(X0 * X0 + X1 * X1 + X2 * X2);
+        const double T = T0 * pow(R, mdelta);
+        const double B = planck_lookup(T, sT, sB, sS, n_t, inv_step);
+        const double phi = scatter ? phase_function(s_los, ux, uy, uz, X0, X1, X2, R, S) : 0.0;
+#pragma unroll
+        for (int j = 0; j < kMaxSlots; ++j) {
+          if (j < cnt) {
+            const int c = first + j;
+            const double* __restrict__ P = M->comp[c];
+            const uint32_t flags = M->comp_flags[c];
+            const double dens = density(M->comp_kind[c], P, flags, X0, X1, X2, R, elon[j]);
+            double val;
+            if (integrand == ZODI_INTEGRAND_KELSALL) {
+              double em = P[ZODI_P_ONE_MINUS_ALBEDO] * (P[ZODI_P_EMISSIVITY] * B);
+              if (flags & ZODI_CF_SCATTER) em += (P[ZODI_P_ALBEDO] * (solar / (R * R))) * phi;
+              val = em * dens;
+            } else {
+              val = B * dens;
+            }
+            acc[j] += wk * val;
+          }
+        }
+      }
+
+      if constexpr (L > 1) {
+#pragma unroll
+        for (int j = 0; j < kMaxSlots; ++j) {
+#pragma unroll
+          for (int off = L / 2; off > 0; off >>= 1) acc[j] += __shfl_xor(acc[j], off, L);
+        }
+      }
+
+#pragma unroll
+      for (int j = 0; j < kMaxSlots; ++j) {
+        if (j < cnt) {
+          const double e = integrand == ZODI_INTEGRAND_KELSALL
+                               ? __dmul_rn(acc[j], h)
+                               : __dmul_rn(__dmul_rn(acc[j], calibration), h);
+          if (a.out_mode == ZODI_OUT_COMPS) {
+            if (active && sub == 0) OUT[(int64_t)(first + j) * n + r] = e;
+          } else {
+            total = __dadd_rn(total, e);
+          }
+        }
+      }
+    }
+    if (a.out_mode == ZODI_OUT_SUM && active && sub == 0) OUT[r] = total;
+  }
+}
+
+// Per-group start/stop distances (zodipy/line_of_sight.py:88-105).
+__global__ __launch_bounds__(256) void line_of_sight_kernel(FwdArgs a, double* __restrict__ start,
+                                                            double* __restrict__ stop) {
+  const zodi_model* __restrict__ M = a.m;
+  const int64_t n = a.n_rays;
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  int64_t s = a.obs_mode == ZODI_OBS_PER_RAY ? r : find_segment(a.seg, a.n_seg, r);
+  const uint32_t bits = a.beyond[a.obs_mode == ZODI_OBS_PER_RAY ? 0 : s];
+  const int64_t os = a.obs_stride;
+  const RayGeom G = ray_geometry(a.u[r], a.u[n + r], a.u[2 * n + r], a.obs[s], a.obs[os + s],
+                                 a.obs[2 * os + s]);
+  for (int g = 0; g < M->n_groups; ++g) {
+    start[(int64_t)g * n + r] =
+        ((bits >> (2 * g)) & 1u) ? kEps : sphere_distance(G, M->group_r_in[g]);
+    stop[(int64_t)g * n + r] =
+        ((bits >> (2 * g + 1)) & 1u) ? kEps : sphere_distance(G, M->group_r_out[g]);
+  }
+}
+
+// Component densities at heliocentric points (zodipy/number_density.py:47-420).
+__global__ __launch_bounds__(256) void number_density_kernel(const zodi_model* __restrict__ M,
+                                                             const double* __restrict__ X,
+                                                             int64_t n, const double* earth,
+                                                             double* __restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double x0 = X[i], x1 = X[n + i], x2 = X[2 * n + i];
+  const double R = sqrt(x0 * x0 + x1 * x1 + x2 * x2);
+  const double ex = earth[0], ey = earth[1];
+  for (int c = 0; c < M->n_comps; ++c) {
+    const double* __restrict__ P = M->comp[c];
+    const int kind = M->comp_kind[c];
+    const double elon = is_feature(kind) ? atan2(ey - P[ZODI_P_Y0], ex - P[ZODI_P_X0]) : 0.0;
+    out[(int64_t)c * n + i] = density(kind, P, M->comp_flags[c], x0, x1, x2, R, elon);
+  }
+}
+
+namespace {
+
+using FwdKernel = void (*)(FwdArgs);
+
+template <int BLOCK>
+FwdKernel pick_lanes(int L) {
+  switch (L) {
+    case 1:
+      return forward_generic<BLOCK, 1>;
+    case 2:
+      return forward_generic<BLOCK, 2>;
+    case 4:
+      return forward_generic<BLOCK, 4>;
+    case 8:
+      return forward_generic<BLOCK, 8>;
+    default:
+      return nullptr;
+  }
+}
+
+FwdKernel pick_forward(int block, int L) {
+  switch (block) {
+    case 64:
+      return pick_lanes<64>(L);
+    case 128:
+      return pick_lanes<128>(L);
+    case 256:
+      return pick_lanes<256>(L);
+    case 512:
+      return pick_lanes<512>(L);
+    default:
+      return nullptr;
+  }
+}
+
+zodi_status fill_args(FwdArgs& a, const zodi_model* model_host, const zodi_model* model_dev,
+                      const double* u, int64_t n_rays, const int64_t* seg_offsets, int32_t n_seg,
+                      const double* obs, const double* earth, int32_t obs_mode,
+                      const uint32_t* beyond) {
+  zodi_status st = validate_model(model_host);
+  if (st != ZODI_OK) return st;
+  if (model_dev == nullptr) return fail(ZODI_ERR_ARG, "model_dev is null");
+  if (n_rays < 0) return fail(ZODI_ERR_ARG, "n_rays %lld < 0", (long long)n_rays);
+  if (u == nullptr || obs == nullptr || earth == nullptr || beyond == nullptr)
+    return fail(ZODI_ERR_ARG, "null input pointer");
+  if (obs_mode == ZODI_OBS_PER_SEGMENT) {
+    if (seg_offsets == nullptr || n_seg < 1)
+      return fail(ZODI_ERR_ARG, "per-segment mode needs seg_offsets and n_seg >= 1");
+  } else if (obs_mode != ZODI_OBS_PER_RAY) {
+    return fail(ZODI_ERR_ARG, "unknown obs_mode %d", obs_mode);
+  }
+  a.m = model_dev;
+  a.u = u;
+  a.n_rays = n_rays;
+  a.seg = seg_offsets;
+  a.n_seg = n_seg;
+  a.obs_mode = obs_mode;
+  a.obs = obs;
+  a.earth = earth;
+  a.obs_stride = obs_mode == ZODI_OBS_PER_RAY ? n_rays : n_seg;
+  a.beyond = beyond;
+  a.out = nullptr;
+  a.out_mode = 0;
+  a.pad = 0;
+  a.n_tiles = 0;
+  return ZODI_OK;
+}
+
+}  // namespace
+}  // namespace zodi
+
+using namespace zodi;
+
+extern "C" {
+
+int32_t zodi_abi_version(void) { return ZODI_ABI_VERSION; }
+
+int64_t zodi_model_size(void) { return (int64_t)sizeof(zodi_model); }
+
+const char* zodi_last_error(void) { return g_last_error.c_str(); }
+
+zodi_status zodi_forward(const zodi_model* model_host, const zodi_model* model_dev,
+                         const double* u, int64_t n_rays, const int64_t* seg_offsets,
+                         int32_t n_seg, const double* obs, const double* earth, int32_t obs_mode,
+                         const uint32_t* beyond, double* out, int32_t out_mode,
+                         const zodi_launch* launch, void* stream) {
+  FwdArgs a;
+  zodi_status st = fill_args(a, model_host, model_dev, u, n_rays, seg_offsets, n_seg, obs, earth,
+                             obs_mode, beyond);
+  if (st != ZODI_OK) return st;
+  if (out == nullptr) return fail(ZODI_ERR_ARG, "out is null");
+  if (out_mode != ZODI_OUT_COMPS && out_mode != ZODI_OUT_SUM)
+    return fail(ZODI_ERR_ARG, "unknown out_mode %d", out_mode);
+  if (n_rays == 0) return ZODI_OK;
+
+  int block = 256, L = 1, per_cu = 0;
+  if (launch != nullptr) {
+    if (launch->block) block = launch->block;
+    if (launch->lanes_per_ray) L = launch->lanes_per_ray;
+    per_cu = launch->blocks_per_cu;
+  }
+  FwdKernel k = pick_forward(block, L);
+  if (k == nullptr)
+    return fail(ZODI_ERR_ARG, "unsupported launch: block %d, lanes_per_ray %d", block, L);
+  if (per_cu < 0 || per_cu > 64) return fail(ZODI_ERR_ARG, "blocks_per_cu %d outside [0, 64]", per_cu);
+
+  const int64_t rays_per_block = block / L;
+  const int64_t n_tiles = (n_rays + rays_per_block - 1) / rays_per_block;
+  int64_t grid = n_tiles;
+  if (per_cu > 0) grid = std::min<int64_t>(grid, (int64_t)per_cu * cu_count());
+  if (grid > 0x7fffffff) return fail(ZODI_ERR_TOO_LARGE, "grid of %lld blocks", (long long)grid);
+  a.out = out;
+  a.out_mode = out_mode;
+  a.n_tiles = n_tiles;
+  hipLaunchKernelGGL(k, dim3((unsigned)grid), dim3(block), 0, (hipStream_t)stream, a);
+  return check_hip(hipGetLastError(), "forward launch");
+}
+
+zodi_status zodi_line_of_sight(const zodi_model* model_host, const zodi_model* model_dev,
+                               const double* u, int64_t n_rays, const int64_t* seg_offsets,
+                               int32_t n_seg, const double* obs, int32_t obs_mode,
+                               const uint32_t* beyond, double* start, double* stop,
+                               void* stream) {
+  FwdArgs a;
+  zodi_status st = fill_args(a, model_host, model_dev, u, n_rays, seg_offsets, n_seg, obs, obs,
+                             obs_mode, beyond);
+  if (st != ZODI_OK) return st;
+  if (start == nullptr || stop == nullptr) return fail(ZODI_ERR_ARG, "null output pointer");
+  if (n_rays == 0) return ZODI_OK;
+  const int64_t grid = (n_rays + 255) / 256;
+  if (grid > 0x7fffffff) return fail(ZODI_ERR_TOO_LARGE, "too many rays");
+  hipLaunchKernelGGL(line_of_sight_kernel, dim3((unsigned)grid), dim3(256), 0,
+                     (hipStream_t)stream, a, start, stop);
+  return check_hip(hipGetLastError(), "line_of_sight launch");
+}
+
+zodi_status zodi_number_density(const zodi_model* model_host, const zodi_model* model_dev,
+                                const double* X, int64_t n_points, const double* earth,
+                                double* out, void* stream) {
+  zodi_status st = validate_model(model_host);
+  if (st != ZODI_OK) return st;
+  if (model_dev == nullptr || X == nullptr || earth == nullptr || out == nullptr)
+    return fail(ZODI_ERR_ARG, "null pointer");
+  if (n_points < 0) return fail(ZODI_ERR_ARG, "n_points < 0");
+  if (n_points == 0) return ZODI_OK;
+  const int64_t grid = (n_points + 255) / 256;
+  if (grid > 0x7fffffff) return fail(ZODI_ERR_TOO_LARGE, "too many points");
+  hipLaunchKernelGGL(number_density_kernel, dim3((unsigned)grid), dim3(256), 0,
+                     (hipStream_t)stream, model_dev, X, n_points, earth, out);
+  return check_hip(hipGetLastError(), "number_density launch");
+}
+
+}  // extern "C"

@@ -1,0 +1,287 @@
+"""Packing of a model into the device model block (``struct zodi_model``, include/zodi_hip.h).
+
+The block is built once per ``Model`` construction or ``update_parameters``
+call and holds everything the kernels read besides the rays themselves:
+
+* component kinds and parameters, with the derived quantities the reference
+  binds into its density partials (sin/cos of ``i`` and ``Omega``, radians,
+  ``3 * n_0`` for the bands; zodipy/component.py:39-44, 87-89, 133-136,
+  zodipy/number_density.py:101) computed with the same NumPy expressions;
+* the effective spectral parameters (emissivity, albedo, ``1 - albedo`` per
+  component; C1-C3, solar irradiance, phase normalisation or the RRM
+  calibration);
+* the line-of-sight cutoffs of each component (zodipy/line_of_sight.py:19-52),
+  grouped into runs of consecutive components sharing cutoffs and temperature
+  law, at most four per group;
+* the Gauss-Legendre nodes/weights and the Planck table with its slopes.
+
+``beyond_bits`` computes the per-observer "outside the cutoff sphere" masks
+(zodipy/line_of_sight.py:71-73) on the host with the reference's expression
+for the observer distance.
+"""
+
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+
+from zodipy_amd.component import (
+    Band,
+    BroadBand,
+    Cloud,
+    Comet,
+    ComponentLabel,
+    Fan,
+    Feature,
+    FeatureRRM,
+    Interstellar,
+    NarrowBand,
+    Ring,
+    RingRRM,
+    ZodiacalComponent,
+)
+from zodipy_amd.cutoffs import COMPONENT_CUTOFFS
+from zodipy_amd.spectral import phase_normalization
+from zodipy_amd.zodiacal_light_model import Integrand, ZodiacalLightModel
+
+ABI_VERSION = 1
+MAX_COMPS = 16
+MAX_GROUPS = 16
+MAX_NODES = 512
+N_TEMPS = 100
+COMP_SLOTS = 24
+SHARED_SLOTS = 8
+MAX_SLOTS_PER_GROUP = 4
+
+# Component kinds (enum in include/zodi_hip.h).
+KIND_CLOUD, KIND_BAND, KIND_RING, KIND_FEATURE = 0, 1, 2, 3
+KIND_FAN, KIND_COMET, KIND_INTERSTELLAR = 4, 5, 6
+KIND_NARROW_BAND, KIND_BROAD_BAND, KIND_RING_RRM, KIND_FEATURE_RRM = 7, 8, 9, 10
+
+CF_CENTERED, CF_SCATTER, CF_P_IS_4 = 1, 2, 4
+
+P_X0, P_Y0, P_Z0, P_NX, P_NY, P_NZ, P_K0 = 0, 1, 2, 3, 4, 5, 6
+P_EMISSIVITY, P_ALBEDO, P_ONE_MINUS_ALBEDO = 20, 21, 22
+S_C1, S_C2, S_C3, S_SOLAR, S_PHASE_NORM, S_CALIBRATION = 0, 1, 2, 3, 4, 5
+
+MODEL_DTYPE = np.dtype(
+    [
+        ("abi_version", "<i4"),
+        ("integrand", "<i4"),
+        ("n_comps", "<i4"),
+        ("n_groups", "<i4"),
+        ("n_nodes", "<i4"),
+        ("n_temps", "<i4"),
+        ("flags", "<i4"),
+        ("reserved", "<i4"),
+        ("comp_kind", "<i4", (MAX_COMPS,)),
+        ("comp_flags", "<u4", (MAX_COMPS,)),
+        ("group_first", "<i4", (MAX_GROUPS,)),
+        ("group_count", "<i4", (MAX_GROUPS,)),
+        ("group_r_in", "<f8", (MAX_GROUPS,)),
+        ("group_r_out", "<f8", (MAX_GROUPS,)),
+        ("group_T0", "<f8", (MAX_GROUPS,)),
+        ("group_delta", "<f8", (MAX_GROUPS,)),
+        ("shared", "<f8", (SHARED_SLOTS,)),
+        ("comp", "<f8", (MAX_COMPS, COMP_SLOTS)),
+        ("gl_x", "<f8", (MAX_NODES,)),
+        ("gl_w", "<f8", (MAX_NODES,)),
+        ("tab_T", "<f8", (N_TEMPS,)),
+        ("tab_B", "<f8", (N_TEMPS,)),
+        ("tab_slope", "<f8", (N_TEMPS,)),
+        ("tab_inv_step", "<f8"),
+        ("pad_", "<f8", (7,)),
+    ]
+)
+
+
+def _kind_of(comp: ZodiacalComponent) -> int:
+    # Subclasses before their parents (RingRRM < Ring, FeatureRRM < Feature).
+    for cls, kind in (
+        (FeatureRRM, KIND_FEATURE_RRM), (RingRRM, KIND_RING_RRM), (Cloud, KIND_CLOUD),
+        (Band, KIND_BAND), (Feature, KIND_FEATURE), (Ring, KIND_RING), (Fan, KIND_FAN),
+        (Comet, KIND_COMET), (Interstellar, KIND_INTERSTELLAR),
+        (NarrowBand, KIND_NARROW_BAND), (BroadBand, KIND_BROAD_BAND),
+    ):
+        if isinstance(comp, cls):
+            return kind
+    msg = f"component type {type(comp).__name__} has no device implementation"
+    raise TypeError(msg)
+
+
+def _kind_slots(comp: ZodiacalComponent, kind: int) -> list[float]:
+    c = comp
+    if kind == KIND_CLOUD:
+        return [c.n_0, c.alpha, c.beta, c.gamma, c.mu]
+    if kind == KIND_BAND:
+        return [3 * c.n_0, c.delta_zeta_rad, c.p, c.v, c.delta_r]
+    if kind == KIND_RING:
+        return [c.n_0, c.R, c.sigma_r, c.sigma_z]
+    if kind == KIND_FEATURE:
+        return [c.n_0, c.R, c.sigma_r, c.sigma_z, c.theta_rad, c.sigma_theta_rad]
+    if kind == KIND_FAN:
+        return [c.Q, c.P, c.gamma, c.Z_0, c.R_outer]
+    if kind == KIND_COMET:
+        return [c.gamma, c.Z_0, c.P, c.amp, c.R_inner, c.R_outer]
+    if kind == KIND_INTERSTELLAR:
+        return [c.amp]
+    if kind == KIND_NARROW_BAND:
+        return [c.beta_nb, c.G, c.gamma, c.A, c.R_inner, c.R_outer]
+    if kind == KIND_BROAD_BAND:
+        return [c.beta_bb, c.sigma_bb, c.gamma, c.A, c.R_inner, c.R_outer]
+    if kind == KIND_RING_RRM:
+        return [c.n_0, c.R, c.sigma_r, c.sigma_z, c.A]
+    if kind == KIND_FEATURE_RRM:
+        return [c.n_0, c.R, c.sigma_r, c.sigma_z, c.theta_rad, c.sigma_theta_rad, c.A]
+    raise AssertionError(kind)
+
+
+@dataclass
+class PackedModel:
+    """Host copy of the device model block plus the metadata the front end needs."""
+
+    block: np.ndarray  # shape (1,), dtype MODEL_DTYPE
+    labels: list[ComponentLabel]
+    group_of_comp: list[int]
+    group_r_in: np.ndarray
+    group_r_out: np.ndarray
+
+    @property
+    def n_comps(self) -> int:
+        return len(self.labels)
+
+    @property
+    def n_groups(self) -> int:
+        return len(self.group_r_in)
+
+    def tobytes(self) -> bytes:
+        return self.block.tobytes()
+
+    def beyond_bits(self, obs_xyz: np.ndarray, any_over_all: bool) -> np.ndarray:
+        """Cutoff masks for observers ``obs_xyz`` (3, n).
+
+        Bit 2g: observer outside group g's start sphere; bit 2g+1: outside its
+        stop sphere. With ``any_over_all`` a single mask is returned for the
+        whole set (one reference ``evaluate`` call covers every observer in it).
+        """
+        obs = np.asarray(obs_xyz, dtype=np.float64).reshape(3, -1)
+        x_0, y_0, z_0 = obs
+        r_obs = np.sqrt(x_0**2 + y_0**2 + z_0**2)
+        bits = np.zeros(r_obs.shape, dtype=np.uint32)
+        for g in range(self.n_groups):
+            bits |= (r_obs > self.group_r_in[g]).astype(np.uint32) << np.uint32(2 * g)
+            bits |= (r_obs > self.group_r_out[g]).astype(np.uint32) << np.uint32(2 * g + 1)
+        if any_over_all:
+            return np.bitwise_or.reduce(bits, keepdims=True).astype(np.uint32)
+        return bits
+
+
+def pack_model(model: ZodiacalLightModel, spectral: dict, b_nu_table: np.ndarray,
+               gl_x: np.ndarray, gl_w: np.ndarray) -> PackedModel:
+    """Build the device block for ``model`` with effective ``spectral`` parameters."""
+    labels = list(model.comps.keys())
+    n_comps = len(labels)
+    if not 1 <= n_comps <= MAX_COMPS:
+        msg = f"models with {n_comps} components are not supported (1..{MAX_COMPS})"
+        raise ValueError(msg)
+    if not 1 <= len(gl_x) <= MAX_NODES:
+        msg = f"gauss_quad_degree {len(gl_x)} outside [1, {MAX_NODES}]"
+        raise ValueError(msg)
+    integrand = model.integrand
+    blk = np.zeros(1, dtype=MODEL_DTYPE)
+    b = blk[0]
+    b["abi_version"] = ABI_VERSION
+    b["integrand"] = int(integrand)
+    b["n_comps"] = n_comps
+    b["n_nodes"] = len(gl_x)
+    b["n_temps"] = N_TEMPS
+    b["gl_x"][: len(gl_x)] = gl_x
+    b["gl_w"][: len(gl_w)] = gl_w
+
+    temps = np.asarray(b_nu_table[0], dtype=np.float64)
+    bnu = np.asarray(b_nu_table[1], dtype=np.float64)
+    if temps.shape != (N_TEMPS,) or bnu.shape != (N_TEMPS,):
+        msg = f"Planck table must be (2, {N_TEMPS})"
+        raise ValueError(msg)
+    b["tab_T"] = temps
+    b["tab_B"] = bnu
+    slope = np.zeros(N_TEMPS)
+    slope[:-1] = (bnu[1:] - bnu[:-1]) / (temps[1:] - temps[:-1])
+    b["tab_slope"] = slope
+    b["tab_inv_step"] = (N_TEMPS - 1) / (temps[-1] - temps[0])
+
+    shared = spectral["shared"]
+    if integrand == Integrand.KELSALL:
+        C1, C2, C3 = shared["C1"], shared["C2"], shared["C3"]
+        b["shared"][S_C1] = C1
+        b["shared"][S_C2] = C2
+        b["shared"][S_C3] = C3
+        b["shared"][S_SOLAR] = shared["solar_irradiance"]
+        any_scatter = any(spectral["comp"][lab]["albedo"] != 0 for lab in labels)
+        b["shared"][S_PHASE_NORM] = phase_normalization(C1, C2, C3) if any_scatter else 0.0
+    else:
+        b["shared"][S_CALIBRATION] = shared["calibration"]
+
+    temps_law = []
+    for c, label in enumerate(labels):
+        comp = model.comps[label]
+        kind = _kind_of(comp)
+        b["comp_kind"][c] = kind
+        flags = 0
+        if comp.x_0 == 0 and comp.y_0 == 0 and comp.z_0 == 0:
+            flags |= CF_CENTERED
+        slots = b["comp"][c]
+        slots[P_X0], slots[P_Y0], slots[P_Z0] = comp.x_0, comp.y_0, comp.z_0
+        slots[P_NX] = comp.sin_Omega_rad * comp.sin_i_rad
+        slots[P_NY] = comp.cos_Omega_rad * comp.sin_i_rad
+        slots[P_NZ] = comp.cos_i_rad
+        ks = _kind_slots(comp, kind)
+        slots[P_K0:P_K0 + len(ks)] = ks
+        if kind == KIND_BAND and comp.p == 4.0:
+            flags |= CF_P_IS_4
+        if integrand == Integrand.KELSALL:
+            sp = spectral["comp"][label]
+            slots[P_EMISSIVITY] = sp["emissivity"]
+            slots[P_ALBEDO] = sp["albedo"]
+            slots[P_ONE_MINUS_ALBEDO] = 1 - sp["albedo"]
+            if sp["albedo"] != 0:
+                flags |= CF_SCATTER
+            temps_law.append((float(shared["T_0"]), float(shared["delta"])))
+        else:
+            sp = spectral["comp"][label]
+            temps_law.append((float(sp["T_0"]), float(sp["delta"])))
+        b["comp_flags"][c] = flags
+
+    # Group consecutive components with identical cutoffs and temperature law.
+    groups: list[list[int]] = []
+    keys = []
+    for c, label in enumerate(labels):
+        r_in, r_out = COMPONENT_CUTOFFS[label]
+        key = (float(r_in), float(r_out), *temps_law[c])
+        if groups and keys[-1] == key and len(groups[-1]) < MAX_SLOTS_PER_GROUP:
+            groups[-1].append(c)
+        else:
+            groups.append([c])
+            keys.append(key)
+    if len(groups) > MAX_GROUPS:
+        msg = f"model needs {len(groups)} component groups (max {MAX_GROUPS})"
+        raise ValueError(msg)
+    b["n_groups"] = len(groups)
+    group_of_comp = [0] * n_comps
+    for g, (members, key) in enumerate(zip(groups, keys)):
+        b["group_first"][g] = members[0]
+        b["group_count"][g] = len(members)
+        b["group_r_in"][g], b["group_r_out"][g] = key[0], key[1]
+        b["group_T0"][g], b["group_delta"][g] = key[2], key[3]
+        for c in members:
+            group_of_comp[c] = g
+    if any(f & CF_SCATTER for f in b["comp_flags"][:n_comps]):
+        b["flags"] = 1
+    return PackedModel(
+        block=blk,
+        labels=labels,
+        group_of_comp=group_of_comp,
+        group_r_in=np.array([k[0] for k in keys]),
+        group_r_out=np.array([k[1] for k in keys]),
+    )

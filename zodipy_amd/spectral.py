@@ -1,0 +1,159 @@
+"""Host-side spectral setup: Planck table, bandpass weights, effective spectral parameters.
+
+Runs once per ``Model`` construction or parameter update; its outputs are
+packed into the device model block. Behaviour follows the reference:
+
+* bandpass weights are normalised by their trapezoid integral over the
+  user's ``x`` in the user's own unit (zodipy/model.py:89-96);
+* the blackbody table holds B_nu(T) in MJy/sr on 100 temperatures
+  linspace(40, 550) K, bandpass-averaged by a trapezoid over the user's ``x``
+  (zodipy/blackbody.py:9-13, 33-49);
+* spectral parameters are interpolated at ``x`` converted to the model's
+  spectral unit: linearly (emissivity, albedo, solar irradiance, RRM
+  calibration) or by nearest neighbour with ties going to the lower node
+  (C1-C3); out-of-range values are linearly extrapolated (nearest: clamped) when
+  extrapolation is enabled; a bandpass is averaged by a trapezoid over ``x`` in
+  the *model's* unit with the weights normalised over the user's unit
+  (zodipy/unpack_model.py:23-173);
+* the RRM calibration is converted from MJy/AU to Jy/cm
+  (zodipy/unpack_model.py:135-136).
+
+The interpolation is implemented here with NumPy index arithmetic; SciPy is
+used only for the trapezoid rule.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+from scipy.integrate import trapezoid
+
+from zodipy_amd import units as u
+from zodipy_amd.zodiacal_light_model import RRM, Kelsall, ZodiacalLightModel
+
+H = 6.62607015e-34  # Planck constant [J s] (exact SI)
+KB = 1.380649e-23  # Boltzmann constant [J/K] (exact SI)
+C = 299792458.0  # speed of light [m/s] (exact SI)
+MJY_SI = 1e-20  # 1 MJy in W m^-2 Hz^-1
+
+TABLE_T_MIN = 40.0
+TABLE_T_MAX = 550.0
+TABLE_N = 100
+
+CALIBRATION_UNIT_FACTOR = 1e6 / (u.AU_SI * 100.0)  # MJy/AU -> Jy/cm
+
+
+def table_temperatures() -> np.ndarray:
+    return np.linspace(TABLE_T_MIN, TABLE_T_MAX, TABLE_N)
+
+
+def normalize_weights(x: u.Quantity, weights) -> np.ndarray | None:
+    if weights is None:
+        return None
+    w = np.asarray(weights, dtype=np.float64)
+    return w / trapezoid(w, np.asarray(x.value, dtype=np.float64))
+
+
+def planck_table(x: u.Quantity, normalized_weights: np.ndarray | None) -> np.ndarray:
+    """Return the (2, 100) array [T; B_nu(T) in MJy/sr] for a delta or a bandpass."""
+    temps = table_temperatures()
+    nu = np.atleast_1d(np.asarray(x.to_value(u.Hz, equivalencies=u.spectral()), dtype=np.float64))
+    a = H * nu[:, None] / (KB * temps[None, :])  # (n_nu, n_T)
+    with np.errstate(over="ignore"):
+        em1 = np.expm1(a)
+    b = (2.0 * H * nu[:, None] ** 3 / (C**2 * em1)) / MJY_SI
+    if normalized_weights is None:
+        values = b[0]
+    else:
+        xv = np.asarray(x.value, dtype=np.float64)
+        values = trapezoid(normalized_weights[:, None] * b, xv, axis=0)
+    return np.stack([temps, values])
+
+
+def _sorted_grid(spectrum: u.Quantity, table):
+    grid = np.asarray(spectrum.value, dtype=np.float64)
+    vals = np.asarray(table, dtype=np.float64)
+    if not np.array_equal(grid, np.sort(grid)):
+        grid, vals = grid[::-1], vals[::-1]
+    return grid, vals
+
+
+def interp_linear(grid: np.ndarray, vals: np.ndarray, xq: np.ndarray) -> np.ndarray:
+    """Piecewise-linear interpolation; the end segments extend beyond the grid."""
+    idx = np.clip(np.searchsorted(grid, xq, side="left"), 1, grid.size - 1)
+    lo, hi = idx - 1, idx
+    slope = (vals[hi] - vals[lo]) / (grid[hi] - grid[lo])
+    return slope * (xq - grid[lo]) + vals[lo]
+
+
+def interp_nearest(grid: np.ndarray, vals: np.ndarray, xq: np.ndarray) -> np.ndarray:
+    """Nearest node; a query exactly half-way between two nodes takes the lower one.
+
+    The half-way points are formed as g[i]/2 + g[i+1]/2 (halve first, then add),
+    which decides ties identically to SciPy's ``interp1d(kind="nearest")``.
+    """
+    half = grid / 2.0
+    mids = half[1:] + half[:-1]
+    idx = np.clip(np.searchsorted(mids, xq, side="left"), 0, grid.size - 1)
+    return vals[idx]
+
+
+def effective_param(x_model: np.ndarray, weights, spectrum: u.Quantity, table,
+                    nearest: bool = False, extrapolate: bool = False):
+    """Spectral parameter at ``x_model`` (already in the spectrum's unit), bandpass-averaged."""
+    grid, vals = _sorted_grid(spectrum, table)
+    xq = np.asarray(x_model, dtype=np.float64)
+    if not extrapolate and (np.any(xq < grid[0]) or np.any(xq > grid[-1])):
+        msg = "A value in x_new is outside the interpolation range."
+        raise ValueError(msg)
+    p = interp_nearest(grid, vals, xq) if nearest else interp_linear(grid, vals, xq)
+    if weights is not None:
+        return trapezoid(np.asarray(weights) * p, x=xq)
+    return p
+
+
+def unpack_spectral(model: ZodiacalLightModel, x: u.Quantity, normalized_weights,
+                    extrapolate: bool) -> dict:
+    """Effective per-component and shared spectral parameters at ``x``.
+
+    Returns ``{"comp": {label: {...}}, "shared": {...}}`` with float values.
+    """
+    xm = np.asarray(x.to_value(model.spectrum.unit, equivalencies=u.spectral()), dtype=np.float64)
+
+    def ev(table, nearest=False):
+        return float(effective_param(xm, normalized_weights, model.spectrum, table,
+                                     nearest=nearest, extrapolate=extrapolate))
+
+    if isinstance(model, Kelsall):
+        comp = {}
+        for label in model.comps:
+            comp[label] = {
+                "emissivity": ev(model.emissivities[label]),
+                "albedo": ev(model.albedos[label]) if model.albedos is not None else 0.0,
+            }
+        shared = {
+            "T_0": float(model.T_0),
+            "delta": float(model.delta),
+            "C1": ev(model.C1, True) if model.C1 is not None else 0.0,
+            "C2": ev(model.C2, True) if model.C2 is not None else 0.0,
+            "C3": ev(model.C3, True) if model.C3 is not None else 0.0,
+            "solar_irradiance": (
+                ev(model.solar_irradiance) if model.solar_irradiance is not None else 0.0
+            ),
+        }
+        return {"comp": comp, "shared": shared}
+    if isinstance(model, RRM):
+        comp = {label: {"T_0": float(model.T_0[label]), "delta": float(model.delta[label])}
+                for label in model.comps}
+        return {"comp": comp,
+                "shared": {"calibration": ev(model.calibration) * CALIBRATION_UNIT_FACTOR}}
+    msg = f"unsupported model type {type(model).__name__}"
+    raise TypeError(msg)
+
+
+def phase_normalization(C1: float, C2: float, C3: float) -> float:
+    """Analytic normalisation N of the phase function N (C1 + C2 Theta + exp(C3 Theta)).
+
+    N = 1 / (2 pi (2 C1 + pi C2 + (exp(pi C3) + 1) / (C3^2 + 1)))
+    (zodipy/scattering.py:53-59).
+    """
+    return 1.0 / (2 * np.pi * (2 * C1 + np.pi * C2 + (np.exp(C3 * np.pi) + 1) / (C3**2 + 1)))
