@@ -1,0 +1,320 @@
+"""Benchmark: batched FP64 zodiacal line-of-sight integral on one MI355X.
+
+Workload (timed): DIRBE model at 12 um, 2000 observers on a circular 1 AU
+ecliptic orbit at seeded uniform phases, each with a seeded pointing count in
+[1000, 50000] (about 51 M rays) of directions uniform on the sphere, 50
+Gauss-Legendre nodes, summed output. One step = one graph-replayed launch over
+the whole batch, inputs resident in HBM.
+
+Also reported (not the headline value):
+* plain: the per-observer drop-in loop (host pointings -> H2D, one launch,
+  D2H, sync, per observer) over the same observers;
+* batched_e2e: the batch including the H2D of all pointings and D2H of the output;
+* fwd_grad_over_fwd: forward + adjoint time over forward time (DIRBE 25 um,
+  nside=128 chi^2 gradient) when the adjoint is available;
+* cpu_baseline: the NumPy oracle with the reference's loop structure and its
+  fork-pool split over host cores, one pool per observer call as the reference
+  creates it, on a bounded prefix of the same observers.
+
+Prints one JSON line.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+if str(ROOT) not in sys.path:
+    sys.path.insert(0, str(ROOT))
+
+SEED = 20261202
+N_OBS = 2000
+COUNT_LO, COUNT_HI = 1000, 50000
+WAVELENGTH_UM = 12.0
+N_NODES = 50
+
+# Algorithmic flop count per ray-node and per ray for the DIRBE model with
+# albedo 0 (frozen convention: +,-,*,/,sqrt = 1, FMA = 2, exp/log/trig = 20,
+# pow = log + exp + 1): per distinct range 23 + 2 transcendentals (3 ranges),
+# cloud 22 + 5, each band 33 + 4, ring 20 + 1, feature 31 + 2, accumulation 4
+# per component; per ray three sphere intersections of 12 + 5.
+FLOP_PER_RAY_NODE_DIRBE = 3 * (23 + 2 * 20) + (22 + 5 * 20) + 3 * (33 + 4 * 20) + (20 + 20) \
+    + (31 + 2 * 20) + 6 * 4  # = 785
+FLOP_PER_RAY_SETUP_DIRBE = 3 * (12 + 5 * 20)  # = 336
+PEAK_FP64_TFLOPS = 78.6
+PEAK_HBM_GBS = 8000.0
+
+
+def log(msg: str) -> None:
+    print(msg, file=sys.stderr, flush=True)
+
+
+def make_observers(seed: int = SEED, n_obs: int = N_OBS):
+    """Observer phases, pointing counts and segment offsets (host, deterministic)."""
+    rng = np.random.default_rng(seed)
+    phases = rng.uniform(0.0, 2.0 * np.pi, n_obs)
+    counts = rng.integers(COUNT_LO, COUNT_HI, n_obs, endpoint=True)
+    seg = np.zeros(n_obs + 1, dtype=np.int64)
+    np.cumsum(counts, out=seg[1:])
+    obs = np.stack([np.cos(phases), np.sin(phases), np.zeros(n_obs)])
+    return seg, obs
+
+
+def make_directions(n: int, device, seed: int = SEED):
+    """(3, n) float64 directions uniform on the sphere, generated on the GPU (seeded)."""
+    import torch
+
+    gen = torch.Generator(device=device)
+    gen.manual_seed(seed)
+    z = torch.rand(n, generator=gen, dtype=torch.float64, device=device).mul_(2.0).sub_(1.0)
+    phi = torch.rand(n, generator=gen, dtype=torch.float64, device=device).mul_(2.0 * np.pi)
+    s = torch.sqrt(1.0 - z * z)
+    return torch.stack([s * torch.cos(phi), s * torch.sin(phi), z])
+
+
+_CPU_MODEL = None
+
+
+def _cpu_eval_chunk(args):
+    u_chunk, obs = args
+    return _CPU_MODEL.evaluate_vectors(u_chunk, obs, obs).sum(axis=0)
+
+
+def cpu_baseline(u, seg, obs, budget_s: float) -> dict:
+    """Reference-structured CPU evaluation of a prefix of the observers.
+
+    The oracle model is built in this process before the pools fork, as the
+    reference's workers inherit its Model (zodipy/model.py:182-198).
+    """
+    import multiprocessing as mp
+    import platform
+
+    from oracle.zodi_oracle import OracleModel
+    from zodipy_amd import units
+
+    global _CPU_MODEL
+    _CPU_MODEL = OracleModel(WAVELENGTH_UM * units.micron)
+
+    nproc = min(len(os.sched_getaffinity(0)), 16)
+    ctx = mp.get_context("fork")
+    done_rays = 0
+    n_done = 0
+    t0 = time.perf_counter()
+    for o in range(len(seg) - 1):
+        a, b = int(seg[o]), int(seg[o + 1])
+        chunks = np.array_split(u[:, a:b], nproc, axis=1)
+        with ctx.Pool(nproc) as pool:  # one pool per evaluate call (zodipy/model.py:191)
+            parts = pool.map(_cpu_eval_chunk, [(c, obs[:, o]) for c in chunks])
+        np.concatenate(parts)
+        done_rays += b - a
+        n_done += 1
+        if time.perf_counter() - t0 > budget_s and n_done >= 3:
+            break
+    dt = time.perf_counter() - t0
+    cpu_model = "unknown"
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                cpu_model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {
+        "value": done_rays * N_NODES / dt,
+        "unit": "ray-node samples/s",
+        "cores": nproc,
+        "kind": "port",
+        "sample": (f"first {n_done} of {len(seg) - 1} observers ({done_rays} rays) of the timed "
+                   f"workload, NumPy oracle, fork pool of {nproc} processes per observer call; "
+                   f"{dt:.1f} s wall on {cpu_model} ({platform.machine()})"),
+    }
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--observers", type=int, default=N_OBS)
+    ap.add_argument("--no-plain", action="store_true", help="skip the per-observer loop")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-grad", action="store_true", help="skip the fwd+grad/fwd ratio")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--retune", action="store_true", help="ignore the tuning cache")
+    ap.add_argument("--profile", action="store_true",
+                    help="short run for profilers: implies --no-plain --no-cpu --no-grad")
+    args = ap.parse_args()
+
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print(f"bench.py: this build measures one GPU only (asked for {max(args.gpus, world)}); "
+              "the batched integral shards by observer with no exchange for the forward pass, "
+              "see the design notes", file=sys.stderr)
+        return 2
+    if args.profile:
+        args.no_plain = args.no_cpu = args.no_grad = True
+
+    import torch
+
+    from zodipy_amd import Model, autotune
+    from zodipy_amd import units as zu
+    from zodipy_amd.batch import BatchedEvaluator
+
+    seg, obs = make_observers(SEED, args.observers)
+    n_rays = int(seg[-1])
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    t_gen = time.perf_counter()
+    u_dev = make_directions(n_rays, dev).contiguous()
+    torch.cuda.synchronize()
+    log(f"workload: {args.observers} observers, {n_rays} rays "
+        f"({time.perf_counter() - t_gen:.2f} s to generate on the GPU)")
+    model = Model(WAVELENGTH_UM * zu.micron, name="dirbe", device=dev)
+    dm = model.device_model
+
+    seg_t, n_seg, obs_t, earth_t, beyond_t, mode = dm.prepare_observers(obs, obs, seg)
+    launch, tune_times = autotune.tune(dm, u_dev, seg_t, n_seg, obs_t, earth_t, beyond_t, mode,
+                                       use_cache=not args.retune, verbose=True)
+    log(f"launch: block {launch.block}, lanes/ray {launch.lanes_per_ray}, "
+        f"blocks/CU {launch.blocks_per_cu}")
+    batch = BatchedEvaluator(dm, u_dev, seg, obs, obs, return_comps=False, launch=launch)
+
+    for _ in range(args.warmup):
+        batch.run()
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ev0.record(stream)
+    for _ in range(args.steps):
+        batch.run()
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms_per_step = ev0.elapsed_time(ev1) / args.steps
+    samples = n_rays * N_NODES
+    value = samples / (ms_per_step / 1e3)
+    log(f"batched: {ms_per_step:.3f} ms/step ({wall / args.steps * 1e3:.3f} ms wall), "
+        f"{value:.4e} ray-node samples/s")
+
+    flops = n_rays * (N_NODES * FLOP_PER_RAY_NODE_DIRBE + FLOP_PER_RAY_SETUP_DIRBE)
+    achieved = flops / (ms_per_step / 1e3) / 1e12
+    traffic = None
+    tfile = ROOT / "profiles" / "forward_batched_dirbe12_traffic.json"
+    if tfile.exists():
+        try:
+            traffic = json.loads(tfile.read_text()).get("hbm_bytes_per_launch")
+        except (OSError, json.JSONDecodeError):
+            traffic = None
+    roofline = {
+        "bound": "mfma",
+        "compute": "FP64 vector ALU (no matrix instructions); peak = MI355X FP64 78.6 TFLOP/s",
+        "achieved": round(achieved, 3),
+        "peak": PEAK_FP64_TFLOPS,
+        "unit": "TFLOP/s",
+        "frac": round(achieved / PEAK_FP64_TFLOPS, 4),
+        "traffic": traffic,
+        "kernel": "forward_layout (DIRBE layout)",
+        "flop_per_launch": flops,
+        "algorithmic_bytes_per_launch": n_rays * (24 + 8),
+        "output_write_GBs": round(n_rays * 8 / (ms_per_step / 1e3) / 1e9, 3),
+    }
+
+    # End-to-end batched: pointings H2D from pinned memory, launch, output D2H.
+    u_pinned = torch.empty((3, n_rays), dtype=torch.float64).pin_memory()
+    u_pinned.copy_(u_dev)
+    u_host = u_pinned.numpy()
+    out_host = torch.empty((n_rays,), dtype=torch.float64).pin_memory()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    u_dev.copy_(u_pinned, non_blocking=True)
+    out = batch.run()
+    out_host.copy_(out, non_blocking=True)
+    torch.cuda.synchronize()
+    e2e = time.perf_counter() - t0
+    batched_e2e = {"seconds": round(e2e, 4), "samples_per_s": samples / e2e}
+    log(f"batched end-to-end (H2D + launch + D2H): {e2e * 1e3:.1f} ms")
+
+    plain = None
+    if not args.no_plain:
+        res = []
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for o in range(len(seg) - 1):
+            a, b = int(seg[o]), int(seg[o + 1])
+            res.append(model.evaluate_vectors(u_host[:, a:b], obspos=obs[:, o],
+                                              earthpos=obs[:, o]).value)
+        t_plain = time.perf_counter() - t0
+        plain = {"seconds": round(t_plain, 4), "samples_per_s": samples / t_plain,
+                 "observers": len(seg) - 1}
+        got = np.concatenate(res)
+        ref = out_host.numpy()
+        plain["max_rel_diff_vs_batched"] = float(np.max(np.abs(got - ref) / np.abs(ref)))
+        log(f"plain per-observer loop: {t_plain:.3f} s, {plain['samples_per_s']:.4e} samples/s; "
+            f"batched is {value / plain['samples_per_s']:.1f}x (kernel-only) and "
+            f"{batched_e2e['samples_per_s'] / plain['samples_per_s']:.1f}x (end to end)")
+
+    cpu = None
+    if not args.no_cpu:
+        log("cpu baseline ...")
+        cpu = cpu_baseline(u_host, seg, obs, args.cpu_seconds)
+        log(f"cpu baseline: {cpu['value']:.3e} samples/s ({cpu['sample']})")
+
+    grad_ratio = None
+    if not args.no_grad:
+        try:
+            from zodipy_amd.fitting import fwd_grad_ratio
+
+            grad_ratio = fwd_grad_ratio(dev)
+            log(f"fwd+grad / fwd: {grad_ratio['ratio']:.3f}")
+        except ImportError:
+            grad_ratio = None
+
+    line = {
+        "metric": "ray-node samples/sec (FP64) batched vs plain, and fwd+grad/fwd time ratio",
+        "value": value,
+        "unit": "ray-node samples/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": ("batched time-ordered: DIRBE 12 um, 2000 observers on a 1 AU circular "
+                         "orbit, 1000-50000 uniform-sphere pointings each, one launch"),
+            "n_rays": n_rays,
+            "n_observers": len(seg) - 1,
+            "gauss_quad_degree": N_NODES,
+            "output": "summed over components",
+            "launch": {"block": launch.block, "lanes_per_ray": launch.lanes_per_ray,
+                       "blocks_per_cu": launch.blocks_per_cu},
+        },
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "plain": plain,
+        "batched_e2e": batched_e2e,
+        "batched_vs_plain": (value / plain["samples_per_s"]) if plain else None,
+        "batched_e2e_vs_plain": (batched_e2e["samples_per_s"] / plain["samples_per_s"])
+        if plain else None,
+        "fwd_grad_over_fwd": grad_ratio["ratio"] if grad_ratio else None,
+        "grad": grad_ratio,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -77,8 +77,10 @@ enum { ZODI_P_X0 = 0, ZODI_P_Y0 = 1, ZODI_P_Z0 = 2,
        ZODI_P_NX = 3,   /* sin(Omega) sin(i) */
        ZODI_P_NY = 4,   /* cos(Omega) sin(i)  (enters with a minus sign) */
        ZODI_P_NZ = 5,   /* cos(i) */
-       ZODI_P_K0 = 6,   /* first kind-specific slot */
-       ZODI_P_EMISSIVITY = 20, ZODI_P_ALBEDO = 21, ZODI_P_ONE_MINUS_ALBEDO = 22 };
+       ZODI_P_K0 = 6,   /* first kind-specific parameter slot (6..12) */
+       ZODI_P_D0 = 13,  /* first kind-specific derived slot (13..19) */
+       ZODI_P_EMISSIVITY = 20, ZODI_P_ALBEDO = 21, ZODI_P_ONE_MINUS_ALBEDO = 22,
+       ZODI_P_SCALE = 23 /* (1 - albedo) * emissivity */ };
 
 /* Shared parameter slots. */
 enum { ZODI_S_C1 = 0, ZODI_S_C2 = 1, ZODI_S_C3 = 2, ZODI_S_SOLAR = 3,
@@ -103,6 +105,14 @@ enum { ZODI_S_C1 = 0, ZODI_S_C2 = 1, ZODI_S_C3 = 2, ZODI_S_SOLAR = 3,
  *   BROAD_BAND   beta_bb, sigma_bb, gamma, A, R_inner, R_outer
  *   RING_RRM     n_0, R, sigma_r, sigma_z, A
  *   FEATURE_RRM  n_0, R, sigma_r, sigma_z, theta_rad, sigma_theta_rad, A
+ * Derived slots, from ZODI_P_D0 (the kernels multiply instead of dividing):
+ *   CLOUD        1/(2 mu), mu/2
+ *   BAND         1/delta_zeta_rad, 1/v, 1/delta_r
+ *   RING(_RRM)   1/sigma_r^2, 1/sigma_z
+ *   FEATURE(_RRM) 1/sigma_r^2, 1/sigma_z, 1/sigma_theta_rad^2
+ *   FAN, COMET   1/Z_0
+ *   NARROW_BAND  1/R_outer
+ *   BROAD_BAND   1/sigma_bb, 1/R_outer
  */
 typedef struct zodi_model {
   int32_t abi_version;

@@ -1,9 +1,12 @@
 // Device functions of the zodiacal line-of-sight integrand (gfx950, FP64 VALU).
 //
 // Everything here is per-lane scalar FP64 code; model parameters are read from
-// the device model block with wave-uniform addresses, so they are fetched by
-// scalar loads into SGPRs. Each function cites the reference expression it
-// evaluates (paths relative to the zodipy 1.1.3 tree).
+// the device model block with wave-uniform addresses (scalar loads into SGPRs).
+// Divisions by parameters use reciprocals precomputed on the host, and each
+// node forms one 1/R shared by its components, so the node loop is free of
+// IEEE FP64 division sequences except for that one. Each function cites the
+// reference expression it evaluates (paths relative to the zodipy 1.1.3 tree);
+// reassociations change results by a few ulp, far below the 1e-10 parity bound.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -21,9 +24,89 @@ constexpr double kRad2Deg = 57.295779513082322865;  // 180.0 / np.pi (np.rad2deg
 
 constexpr int kMaxSlots = 4;  // components per group held in registers
 
+// Parameter block accessor: loads through the constant address space are
+// known not to be written by the kernel, so wave-uniform loads become scalar.
+typedef const __attribute__((address_space(4))) zodi_model* ModelPtr;
+typedef const __attribute__((address_space(4))) double* ParamPtr;
+
+__device__ __forceinline__ ModelPtr as_const(const zodi_model* m) { return (ModelPtr)m; }
+
+// The same pointer, made opaque to the optimizer (an empty asm statement, no
+// instruction): loads through it inside a loop are not hoisted out of it, so
+// the many uniform parameters are re-read by scalar loads each iteration
+// instead of being held across the loop, which would overflow the SGPR budget
+// into VGPRs.
+__device__ __forceinline__ ModelPtr opaque(ModelPtr m) {
+  uint64_t v = (uint64_t)m;
+  asm volatile("" : "+s"(v));
+  return (ModelPtr)v;
+}
+
 // np.maximum: returns the first argument when it is NaN, propagates NaN.
 __device__ __forceinline__ double np_maximum(double a, double b) {
   return (a >= b || a != a) ? a : b;
+}
+
+// Natural logarithm with a few ulp of error (the kernels need ~1e-14).
+// x = 2^e m with m in [sqrt(1/2), sqrt(2)); log(m) = 2 atanh(s), s = (m-1)/(m+1),
+// |s| <= 0.1716, summed as 2s + 2s^3 (1/3 + s^2/5 + ... + s^20/21) (truncation
+// below 1e-17); e ln 2 is added in two parts (ln2_hi has trailing zero bits so
+// e * ln2_hi is exact). 0 -> -inf, negative or NaN -> NaN, +inf -> +inf.
+__device__ __forceinline__ double log_fast(double x) {
+  constexpr double kSqrtHalf = 0.70710678118654752440;
+  constexpr double kLn2Hi = 6.93147180369123816490e-01;  // 0x3FE62E42FEE00000
+  constexpr double kLn2Lo = 1.90821492927058770002e-10;  // ln 2 - kLn2Hi
+  int e = __builtin_amdgcn_frexp_exp(x);
+  double m = __builtin_amdgcn_frexp_mant(x);
+  if (m < kSqrtHalf) {
+    m += m;
+    e -= 1;
+  }
+  const double f = m - 1.0;  // exact
+  const double s = f / (2.0 + f);
+  const double z = s * s;
+  double p = 1.0 / 21.0;
+  p = p * z + 1.0 / 19.0;
+  p = p * z + 1.0 / 17.0;
+  p = p * z + 1.0 / 15.0;
+  p = p * z + 1.0 / 13.0;
+  p = p * z + 1.0 / 11.0;
+  p = p * z + 1.0 / 9.0;
+  p = p * z + 1.0 / 7.0;
+  p = p * z + 1.0 / 5.0;
+  p = p * z + 1.0 / 3.0;
+  const double two_s = s + s;
+  const double log_m = two_s + two_s * (z * p);
+  const double de = (double)e;
+  const double r = de * kLn2Hi + (de * kLn2Lo + log_m);
+  if (x > 0.0 && x < __builtin_inf()) return r;
+  if (x == 0.0) return -__builtin_inf();
+  return x == __builtin_inf() ? x : __builtin_nan("");
+}
+
+// x^y for x >= 0 (every power in the model has a non-negative base) as
+// exp(y log x). The result differs from a correctly rounded pow by about
+// |y log x| ulp; x = 0 gives 0 for y > 0; y = 0 gives 1 like pow.
+__device__ __forceinline__ double pow_pos(double x, double y) {
+  return y == 0.0 ? 1.0 : exp(y * log_fast(x));
+}
+
+// |X| and 1/|X| from |X|^2: a hardware reciprocal square root refined by two
+// Newton steps (y <- y (3 - a y^2) / 2), then |X| = a y. Both within ~1 ulp.
+__device__ __forceinline__ void norm_and_inverse(double a, double& r, double& inv_r) {
+  double y = __builtin_amdgcn_rsq(a);
+  double t = a * y;
+  double h = 0.5 * y;
+  double e = fma(-t, h, 0.5);
+  y = fma(y, e, y);  // refined 1/sqrt(a)
+  t = fma(t, e, t);  // refined sqrt(a)
+  h = 0.5 * y;
+  const double d = fma(-t, t, a);
+  t = fma(d, h, t);  // sqrt(a), last Newton correction
+  e = fma(-t, y, 1.0);
+  y = fma(y, e, y);
+  r = t;
+  inv_r = y;
 }
 
 // Python/NumPy float remainder a % b for b > 0: fmod, then shift negative
@@ -92,38 +175,43 @@ __device__ __forceinline__ double planck_lookup(double T, const double* __restri
   return tS[j] * (T - Tj) + tB[j];
 }
 
-// Heliocentric position in a component's frame: offset by X_0 unless the
-// component is centred on the Sun (then it is exactly the heliocentric one).
+// Position in a component's frame: offset by X_0 unless the component is
+// centred on the Sun (then it is exactly the heliocentric one, and so are |X|
+// and 1/|X|). zc is the height above the symmetry plane:
+// Z = X'_x sin(Omega) sin(i) - X'_y cos(Omega) sin(i) + X'_z cos(i).
 struct CompFrame {
-  double x, y, z, r, zc;  // offset position, its norm, height above the symmetry plane
+  double x, y, z, r, inv_r, zc;
 };
 
-__device__ __forceinline__ CompFrame comp_frame(const double* __restrict__ P, uint32_t flags,
-                                                double X0, double X1, double X2, double R) {
+__device__ __forceinline__ CompFrame comp_frame(ParamPtr P, uint32_t flags, double X0, double X1,
+                                                double X2, double R, double inv_R) {
   CompFrame f;
   if (flags & ZODI_CF_CENTERED) {
     f.x = X0;
     f.y = X1;
     f.z = X2;
     f.r = R;
+    f.inv_r = inv_R;
   } else {
     f.x = X0 - P[ZODI_P_X0];
     f.y = X1 - P[ZODI_P_Y0];
     f.z = X2 - P[ZODI_P_Z0];
-    f.r = sqrt(f.x * f.x + f.y * f.y + f.z * f.z);
+    norm_and_inverse(f.x * f.x + f.y * f.y + f.z * f.z, f.r, f.inv_r);
   }
-  // Z = X'_x sin(Omega) sin(i) - X'_y cos(Omega) sin(i) + X'_z cos(i)
   f.zc = f.x * P[ZODI_P_NX] - f.y * P[ZODI_P_NY] + f.z * P[ZODI_P_NZ];
   return f;
 }
 
 // ---------------------------------------------------------------- densities
-// zodipy/number_density.py:47-73
-__device__ __forceinline__ double cloud_density(const double* __restrict__ P, const CompFrame& f) {
+// Cloud, zodipy/number_density.py:47-73:
+// n_0 R^-alpha exp(-beta g^gamma), g = zeta^2/(2 mu) if zeta < mu else zeta - mu/2,
+// evaluated as n_0 exp(-alpha log R - beta g^gamma).
+__device__ __forceinline__ double cloud_density(ParamPtr P, const CompFrame& f) {
   const double n0 = P[6], alpha = P[7], beta = P[8], gamma = P[9], mu = P[10];
-  const double zeta = fabs(f.zc / f.r);
-  const double g = zeta < mu ? (zeta * zeta) / (2.0 * mu) : zeta - mu / 2.0;
-  return n0 * pow(f.r, -alpha) * exp(-beta * pow(g, gamma));
+  const double inv_2mu = P[13], half_mu = P[14];
+  const double zeta = fabs(f.zc) * f.inv_r;
+  const double g = zeta < mu ? (zeta * zeta) * inv_2mu : zeta - half_mu;
+  return n0 * exp(-alpha * log_fast(f.r) - beta * pow_pos(g, gamma));
 }
 
 __device__ __forceinline__ double pow20(double x) {
@@ -134,96 +222,97 @@ __device__ __forceinline__ double pow20(double x) {
   return x16 * x4;
 }
 
-// zodipy/number_density.py:76-110 (P[6] holds 3*n_0; the 1/v factor as in the reference)
-__device__ __forceinline__ double band_density(const double* __restrict__ P, uint32_t flags,
-                                               const CompFrame& f) {
-  const double three_n0 = P[6], dzeta = P[7], p = P[8], v = P[9], dr = P[10];
-  const double zeta = fabs(f.zc / f.r);
-  const double zd = zeta / dzeta;
+// Band, zodipy/number_density.py:76-110 (P[6] holds 3 n_0; the 1/v factor as in
+// the reference): 3 n_0/R exp(-zd^6) (1 + zd^p / v) (1 - exp(-(R/delta_r)^20)),
+// zd = zeta / delta_zeta.
+__device__ __forceinline__ double band_density(ParamPtr P, uint32_t flags, const CompFrame& f) {
+  const double three_n0 = P[6], p = P[8];
+  const double inv_dzeta = P[13], inv_v = P[14], inv_dr = P[15];
+  const double zd = (fabs(f.zc) * f.inv_r) * inv_dzeta;
   const double zd2 = zd * zd;
   const double zd4 = zd2 * zd2;
   const double zd6 = zd4 * zd2;
-  const double term1 = three_n0 / f.r;
-  const double term2 = exp(-zd6);
-  const double zdp = (flags & ZODI_CF_P_IS_4) ? zd4 : pow(zd, p);
-  const double term3 = 1.0 + zdp / v;
-  const double term4 = 1.0 - exp(-pow20(f.r / dr));
-  return term1 * term2 * term3 * term4;
+  const double zdp = (flags & ZODI_CF_P_IS_4) ? zd4 : pow_pos(zd, p);
+  const double term3 = 1.0 + zdp * inv_v;
+  const double term4 = 1.0 - exp(-pow20(f.r * inv_dr));
+  return (three_n0 * f.inv_r) * exp(-zd6) * term3 * term4;
 }
 
-// zodipy/number_density.py:113-139
-__device__ __forceinline__ double ring_density(const double* __restrict__ P, const CompFrame& f) {
-  const double n0 = P[6], Rr = P[7], sr = P[8], sz = P[9];
+// Ring, zodipy/number_density.py:113-139: n_0 exp(-(R - R_r)^2/sigma_r^2 - |Z|/sigma_z).
+__device__ __forceinline__ double ring_density(ParamPtr P, const CompFrame& f) {
+  const double n0 = P[6], Rr = P[7];
   const double d = f.r - Rr;
-  const double term1 = -(d * d) / (sr * sr);
-  const double term2 = fabs(f.zc) / sz;
-  return n0 * exp(term1 - term2);
+  return n0 * exp(-(d * d) * P[13] - fabs(f.zc) * P[14]);
 }
 
-// zodipy/number_density.py:142-181; `earth_lon` = atan2(E'_y, E'_x) with E' = X_earth - X_0.
-__device__ __forceinline__ double feature_density(const double* __restrict__ P, const CompFrame& f,
+// Feature, zodipy/number_density.py:142-181: n_0 exp(-((R - R_f)^2/sigma_r^2 +
+// |Z|/sigma_z + dtheta^2/sigma_theta^2)), dtheta wrapped with a floor modulo;
+// `earth_lon` = atan2(E'_y, E'_x) with E' = X_earth - X_0.
+__device__ __forceinline__ double feature_density(ParamPtr P, const CompFrame& f,
                                                   double earth_lon) {
-  const double n0 = P[6], Rf = P[7], sr = P[8], sz = P[9], th = P[10], sth = P[11];
+  const double n0 = P[6], Rf = P[7], th = P[10];
   const double theta_comp = atan2(f.y, f.x) - earth_lon;
   double dth = theta_comp - th;
   dth = floor_mod_pos(dth + kPi, kTwoPi) - kPi;
   const double d = f.r - Rf;
-  double e = (d * d) / (sr * sr);
-  e += fabs(f.zc) / sz;
-  e += (dth * dth) / (sth * sth);
+  double e = (d * d) * P[13];
+  e += fabs(f.zc) * P[14];
+  e += (dth * dth) * P[15];
   return n0 * exp(-e);
 }
 
-// zodipy/number_density.py:184-218 (zero outside R <= R_outer)
-__device__ __forceinline__ double fan_density(const double* __restrict__ P, const CompFrame& f) {
+// Fan, zodipy/number_density.py:184-218 (zero outside R <= R_outer).
+__device__ __forceinline__ double fan_density(ParamPtr P, const CompFrame& f) {
   const double Q = P[6], Pp = P[7], gamma = P[8], Z0 = P[9], Rout = P[10];
   if (!(f.r <= Rout)) return 0.0;
-  const double beta = asin(f.zc / f.r);
+  const double beta = asin(f.zc * f.inv_r);
   const double za = fabs(f.zc);
-  const double eps = za < Z0 ? 2.0 - za / Z0 : 1.0;
-  const double fv = pow(cos(beta), Q) * exp(-Pp * sin(pow(fabs(beta), eps)));
-  return pow(f.r, -gamma) * fv;
+  const double eps = za < Z0 ? 2.0 - za * P[13] : 1.0;
+  const double fv = pow_pos(cos(beta), Q) * exp(-Pp * sin(pow_pos(fabs(beta), eps)));
+  return pow_pos(f.r, -gamma) * fv;
 }
 
-// zodipy/number_density.py:221-256 (zero outside R_inner <= R <= R_outer)
-__device__ __forceinline__ double comet_density(const double* __restrict__ P, const CompFrame& f) {
+// Comet, zodipy/number_density.py:221-256 (zero outside R_inner <= R <= R_outer).
+__device__ __forceinline__ double comet_density(ParamPtr P, const CompFrame& f) {
   const double gamma = P[6], Z0 = P[7], Pp = P[8], amp = P[9], Rin = P[10], Rout = P[11];
   if (!(f.r >= Rin && f.r <= Rout)) return 0.0;
-  const double beta = asin(f.zc / f.r);
+  const double beta = asin(f.zc * f.inv_r);
   const double za = fabs(f.zc);
-  const double eps = za < Z0 ? 2.0 - za / Z0 : 1.0;
-  const double fv = exp(-Pp * sin(pow(fabs(beta), eps)));
-  return amp * pow(f.r, -gamma) * fv;
+  const double eps = za < Z0 ? 2.0 - za * P[13] : 1.0;
+  const double fv = exp(-Pp * sin(pow_pos(fabs(beta), eps)));
+  return amp * pow_pos(f.r, -gamma) * fv;
 }
 
-// zodipy/number_density.py:267-304 (latitude in degrees)
-__device__ __forceinline__ double narrow_band_density(const double* __restrict__ P,
-                                                      const CompFrame& f) {
+// Narrow band, zodipy/number_density.py:267-304 (latitude in degrees).
+__device__ __forceinline__ double narrow_band_density(ParamPtr P, const CompFrame& f) {
   const double bnb = P[6], G = P[7], gamma = P[8], A = P[9], Rin = P[10], Rout = P[11];
   if (!(f.r >= Rin && f.r <= Rout)) return 0.0;
-  const double babs = fabs(asin(f.zc / f.r) * kRad2Deg);
+  const double babs = fabs(asin(f.zc * f.inv_r) * kRad2Deg);
   const double fv = babs < bnb ? exp(G * (babs - bnb)) : 0.0;
-  return A * pow(f.r / Rout, -gamma) * fv;
+  return A * pow_pos(f.r * P[13], -gamma) * fv;
 }
 
-// zodipy/number_density.py:307-342 (latitude in degrees)
-__device__ __forceinline__ double broad_band_density(const double* __restrict__ P,
-                                                     const CompFrame& f) {
-  const double bbb = P[6], sbb = P[7], gamma = P[8], A = P[9], Rin = P[10], Rout = P[11];
+// Broad band, zodipy/number_density.py:307-342 (latitude in degrees).
+__device__ __forceinline__ double broad_band_density(ParamPtr P, const CompFrame& f) {
+  const double bbb = P[6], gamma = P[8], A = P[9], Rin = P[10], Rout = P[11];
   if (!(f.r >= Rin && f.r <= Rout)) return 0.0;
-  const double beta = asin(f.zc / f.r) * kRad2Deg;
-  const double a = (beta - bbb) / sbb;
-  const double c = (beta + bbb) / sbb;
+  const double beta = asin(f.zc * f.inv_r) * kRad2Deg;
+  const double a = (beta - bbb) * P[13];
+  const double c = (beta + bbb) * P[13];
   const double fv = exp(-0.5 * (a * a)) + exp(-0.5 * (c * c));
-  return A * pow(f.r / Rout, -gamma) * fv;
+  return A * pow_pos(f.r * P[14], -gamma) * fv;
 }
 
-// Density of component `kind` at heliocentric X (|X| = R).
-__device__ __forceinline__ double density(int kind, const double* __restrict__ P, uint32_t flags,
-                                          double X0, double X1, double X2, double R,
+__device__ __forceinline__ bool is_feature(int kind) {
+  return kind == ZODI_FEATURE || kind == ZODI_FEATURE_RRM;
+}
+
+// Density of component `kind` at heliocentric X (|X| = R, 1/|X| = inv_R).
+__device__ __forceinline__ double density(int kind, ParamPtr P, uint32_t flags, double X0,
+                                          double X1, double X2, double R, double inv_R,
                                           double earth_lon) {
   if (kind == ZODI_INTERSTELLAR) return P[6];  // zodipy/number_density.py:259-264
-  const CompFrame f = comp_frame(P, flags, X0, X1, X2, R);
+  const CompFrame f = comp_frame(P, flags, X0, X1, X2, R, inv_R);
   switch (kind) {
     case ZODI_CLOUD:
       return cloud_density(P, f);
@@ -250,16 +339,12 @@ __device__ __forceinline__ double density(int kind, const double* __restrict__ P
   }
 }
 
-__device__ __forceinline__ bool is_feature(int kind) {
-  return kind == ZODI_FEATURE || kind == ZODI_FEATURE_RRM;
-}
-
 // Scattering phase function at the node (zodipy/scattering.py:11-50):
 // Theta = arccos(-clip(X_los . X_helio / (R_los R_helio), -1, 1)),
 // Phi = N (C1 + C2 Theta + exp(C3 Theta)).
 __device__ __forceinline__ double phase_function(double R_los, double ux, double uy, double uz,
                                                  double X0, double X1, double X2, double R,
-                                                 const double* __restrict__ S) {
+                                                 ParamPtr S) {
   const double lx = R_los * ux, ly = R_los * uy, lz = R_los * uz;
   double ct = (lx * X0 + ly * X1 + lz * X2) / (R_los * R);
   ct = ct != ct ? ct : (ct < -1.0 ? -1.0 : (ct > 1.0 ? 1.0 : ct));

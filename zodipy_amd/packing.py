@@ -61,8 +61,8 @@ KIND_NARROW_BAND, KIND_BROAD_BAND, KIND_RING_RRM, KIND_FEATURE_RRM = 7, 8, 9, 10
 
 CF_CENTERED, CF_SCATTER, CF_P_IS_4 = 1, 2, 4
 
-P_X0, P_Y0, P_Z0, P_NX, P_NY, P_NZ, P_K0 = 0, 1, 2, 3, 4, 5, 6
-P_EMISSIVITY, P_ALBEDO, P_ONE_MINUS_ALBEDO = 20, 21, 22
+P_X0, P_Y0, P_Z0, P_NX, P_NY, P_NZ, P_K0, P_D0 = 0, 1, 2, 3, 4, 5, 6, 13
+P_EMISSIVITY, P_ALBEDO, P_ONE_MINUS_ALBEDO, P_SCALE = 20, 21, 22, 23
 S_C1, S_C2, S_C3, S_SOLAR, S_PHASE_NORM, S_CALIBRATION = 0, 1, 2, 3, 4, 5
 
 MODEL_DTYPE = np.dtype(
@@ -108,6 +108,26 @@ def _kind_of(comp: ZodiacalComponent) -> int:
             return kind
     msg = f"component type {type(comp).__name__} has no device implementation"
     raise TypeError(msg)
+
+
+def _derived_slots(comp: ZodiacalComponent, kind: int) -> list[float]:
+    """Reciprocals and constants the kernels multiply by instead of dividing."""
+    c = comp
+    if kind == KIND_CLOUD:
+        return [1.0 / (2 * c.mu), c.mu / 2]
+    if kind == KIND_BAND:
+        return [1.0 / c.delta_zeta_rad, 1.0 / c.v, 1.0 / c.delta_r]
+    if kind in (KIND_RING, KIND_RING_RRM):
+        return [1.0 / c.sigma_r**2, 1.0 / c.sigma_z]
+    if kind in (KIND_FEATURE, KIND_FEATURE_RRM):
+        return [1.0 / c.sigma_r**2, 1.0 / c.sigma_z, 1.0 / c.sigma_theta_rad**2]
+    if kind in (KIND_FAN, KIND_COMET):
+        return [1.0 / c.Z_0]
+    if kind == KIND_NARROW_BAND:
+        return [1.0 / c.R_outer]
+    if kind == KIND_BROAD_BAND:
+        return [1.0 / c.sigma_bb, 1.0 / c.R_outer]
+    return []
 
 
 def _kind_slots(comp: ZodiacalComponent, kind: int) -> list[float]:
@@ -238,6 +258,8 @@ def pack_model(model: ZodiacalLightModel, spectral: dict, b_nu_table: np.ndarray
         slots[P_NZ] = comp.cos_i_rad
         ks = _kind_slots(comp, kind)
         slots[P_K0:P_K0 + len(ks)] = ks
+        ds = _derived_slots(comp, kind)
+        slots[P_D0:P_D0 + len(ds)] = ds
         if kind == KIND_BAND and comp.p == 4.0:
             flags |= CF_P_IS_4
         if integrand == Integrand.KELSALL:
@@ -245,6 +267,7 @@ def pack_model(model: ZodiacalLightModel, spectral: dict, b_nu_table: np.ndarray
             slots[P_EMISSIVITY] = sp["emissivity"]
             slots[P_ALBEDO] = sp["albedo"]
             slots[P_ONE_MINUS_ALBEDO] = 1 - sp["albedo"]
+            slots[P_SCALE] = (1 - sp["albedo"]) * sp["emissivity"]
             if sp["albedo"] != 0:
                 flags |= CF_SCATTER
             temps_law.append((float(shared["T_0"]), float(shared["delta"])))
