@@ -148,6 +148,8 @@ def main() -> int:
     ap.add_argument("--no-grad", action="store_true", help="skip the fwd+grad/fwd ratio")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--retune", action="store_true", help="ignore the tuning cache")
+    ap.add_argument("--launch", default=None,
+                    help="fixed launch 'block,lanes_per_ray,blocks_per_cu' instead of tuning")
     ap.add_argument("--profile", action="store_true",
                     help="short run for profilers: implies --no-plain --no-cpu --no-grad")
     args = ap.parse_args()
@@ -180,8 +182,14 @@ def main() -> int:
     dm = model.device_model
 
     seg_t, n_seg, obs_t, earth_t, beyond_t, mode = dm.prepare_observers(obs, obs, seg)
-    launch, tune_times = autotune.tune(dm, u_dev, seg_t, n_seg, obs_t, earth_t, beyond_t, mode,
-                                       use_cache=not args.retune, verbose=True)
+    if args.launch:
+        from zodipy_amd._lib import Launch
+
+        b, lanes, per_cu = (int(v) for v in args.launch.split(","))
+        launch = Launch(b, lanes, per_cu, autotune.VARIANT_AUTO)
+    else:
+        launch, _ = autotune.tune(dm, u_dev, seg_t, n_seg, obs_t, earth_t, beyond_t, mode,
+                                  use_cache=not args.retune, verbose=True)
     log(f"launch: block {launch.block}, lanes/ray {launch.lanes_per_ray}, "
         f"blocks/CU {launch.blocks_per_cu}")
     batch = BatchedEvaluator(dm, u_dev, seg, obs, obs, return_comps=False, launch=launch)

@@ -20,6 +20,9 @@ from zodipy_amd import _lib
 from zodipy_amd._lib import Launch
 from zodipy_amd.runtime import OUT_SUM, DeviceModel
 
+# Kernel variant: 1 = layout-specialised kernel when the model layout has one, else generic.
+VARIANT_AUTO = 1
+
 CANDIDATES = [
     (b, lanes, per_cu)
     for b in (64, 128, 256, 512)
@@ -42,7 +45,8 @@ def layout_key(dm: DeviceModel, n_rays: int) -> str:
     groups = ",".join(str(int(c)) for c in b["group_count"][: int(b["n_groups"])])
     name = torch.cuda.get_device_name(dm.device).replace(" ", "_")
     bucket = int(np.ceil(np.log2(max(n_rays, 1))))
-    return f"{name}|i{int(b['integrand'])}|k{kinds}|g{groups}|n{int(b['n_nodes'])}|b{bucket}"
+    return (f"{name}|v{VARIANT_AUTO}|i{int(b['integrand'])}|k{kinds}|g{groups}"
+            f"|n{int(b['n_nodes'])}|b{bucket}")
 
 
 def _load() -> dict:
@@ -90,7 +94,7 @@ def tune(dm: DeviceModel, u: torch.Tensor, seg, n_seg: int, obs, earth, beyond, 
     cache = _load() if use_cache else {}
     if use_cache and key in cache:
         b, lanes, per_cu = cache[key]["best"]
-        return Launch(b, lanes, per_cu, 0), cache[key].get("times", {})
+        return Launch(b, lanes, per_cu, VARIANT_AUTO), cache[key].get("times", {})
 
     # Time on a prefix of the batch (whole segments) to bound tuning cost.
     if n_rays > max_rays and seg is not None:
@@ -111,7 +115,7 @@ def tune(dm: DeviceModel, u: torch.Tensor, seg, n_seg: int, obs, earth, beyond, 
     out = torch.empty((n_sub,), dtype=torch.float64, device=u.device)
     times = {}
     for b, lanes, per_cu in candidates or CANDIDATES:
-        launch = Launch(b, lanes, per_cu, 0)
+        launch = Launch(b, lanes, per_cu, VARIANT_AUTO)
 
         def fn(launch=launch):
             st = dm.lib.zodi_forward(
@@ -131,4 +135,4 @@ def tune(dm: DeviceModel, u: torch.Tensor, seg, n_seg: int, obs, earth, beyond, 
     if use_cache:
         cache[key] = {"best": [b, lanes, per_cu], "times": times}
         _save(cache)
-    return Launch(b, lanes, per_cu, 0), times
+    return Launch(b, lanes, per_cu, VARIANT_AUTO), times

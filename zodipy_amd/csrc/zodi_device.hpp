@@ -13,6 +13,7 @@
 #include <stdint.h>
 
 #include "zodi_hip.h"
+#include "zodi_math.hpp"
 
 namespace zodi {
 
@@ -47,48 +48,16 @@ __device__ __forceinline__ double np_maximum(double a, double b) {
   return (a >= b || a != a) ? a : b;
 }
 
-// Natural logarithm with a few ulp of error (the kernels need ~1e-14).
-// x = 2^e m with m in [sqrt(1/2), sqrt(2)); log(m) = 2 atanh(s), s = (m-1)/(m+1),
-// |s| <= 0.1716, summed as 2s + 2s^3 (1/3 + s^2/5 + ... + s^20/21) (truncation
-// below 1e-17); e ln 2 is added in two parts (ln2_hi has trailing zero bits so
-// e * ln2_hi is exact). 0 -> -inf, negative or NaN -> NaN, +inf -> +inf.
-__device__ __forceinline__ double log_fast(double x) {
-  constexpr double kSqrtHalf = 0.70710678118654752440;
-  constexpr double kLn2Hi = 6.93147180369123816490e-01;  // 0x3FE62E42FEE00000
-  constexpr double kLn2Lo = 1.90821492927058770002e-10;  // ln 2 - kLn2Hi
-  int e = __builtin_amdgcn_frexp_exp(x);
-  double m = __builtin_amdgcn_frexp_mant(x);
-  if (m < kSqrtHalf) {
-    m += m;
-    e -= 1;
-  }
-  const double f = m - 1.0;  // exact
-  const double s = f / (2.0 + f);
-  const double z = s * s;
-  double p = 1.0 / 21.0;
-  p = p * z + 1.0 / 19.0;
-  p = p * z + 1.0 / 17.0;
-  p = p * z + 1.0 / 15.0;
-  p = p * z + 1.0 / 13.0;
-  p = p * z + 1.0 / 11.0;
-  p = p * z + 1.0 / 9.0;
-  p = p * z + 1.0 / 7.0;
-  p = p * z + 1.0 / 5.0;
-  p = p * z + 1.0 / 3.0;
-  const double two_s = s + s;
-  const double log_m = two_s + two_s * (z * p);
-  const double de = (double)e;
-  const double r = de * kLn2Hi + (de * kLn2Lo + log_m);
-  if (x > 0.0 && x < __builtin_inf()) return r;
-  if (x == 0.0) return -__builtin_inf();
-  return x == __builtin_inf() ? x : __builtin_nan("");
-}
-
 // x^y for x >= 0 (every power in the model has a non-negative base) as
-// exp(y log x). The result differs from a correctly rounded pow by about
+// exp_fast(y log x). The result differs from a correctly rounded pow by about
 // |y log x| ulp; x = 0 gives 0 for y > 0; y = 0 gives 1 like pow.
 __device__ __forceinline__ double pow_pos(double x, double y) {
-  return y == 0.0 ? 1.0 : exp(y * log_fast(x));
+  return y == 0.0 ? 1.0 : exp_fast(y * log_fast(x));
+}
+
+// x^y for a strictly positive finite base (radii).
+__device__ __forceinline__ double pow_radius(double x, double y) {
+  return y == 0.0 ? 1.0 : exp_fast(y * log_pos(x));
 }
 
 // |X| and 1/|X| from |X|^2: a hardware reciprocal square root refined by two
@@ -109,10 +78,20 @@ __device__ __forceinline__ void norm_and_inverse(double a, double& r, double& in
   inv_r = y;
 }
 
-// Python/NumPy float remainder a % b for b > 0: fmod, then shift negative
-// results into [0, b) (numpy npy_divmod). fmod is exact.
+// Python/NumPy float remainder a % b for b > 0 (numpy npy_divmod): the exact
+// fmod(a, b), then negative results shifted by +b (one rounding) and zero made +0.
+// For |a| < 4b fmod is formed without a loop: subtracting 2b from |a| in [2b, 4b)
+// and b from |a| in [b, 2b) are both exact (Sterbenz), so the result equals fmod's.
 __device__ __forceinline__ double floor_mod_pos(double a, double b) {
-  double m = fmod(a, b);
+  double m;
+  if (fabs(a) < 4.0 * b) {
+    m = a;
+    const double two_b = 2.0 * b;
+    if (fabs(m) >= two_b) m = m > 0.0 ? m - two_b : m + two_b;
+    if (fabs(m) >= b) m = m > 0.0 ? m - b : m + b;
+  } else {
+    m = fmod(a, b);
+  }
   if (m < 0.0) {
     m += b;
   } else if (m == 0.0) {
@@ -152,27 +131,20 @@ __device__ __forceinline__ double sphere_distance(const RayGeom& g, double cutof
   return np_maximum(q, c / q);
 }
 
-// Linear interpolation in the Planck table with np.interp semantics: NaN in,
-// NaN out; clamped to the end values outside [T0, T_last]; exact table values
-// at the nodes. Tables live in LDS.
-__device__ __forceinline__ double planck_lookup(double T, const double* __restrict__ tT,
-                                                const double* __restrict__ tB,
-                                                const double* __restrict__ tS, int n,
-                                                double inv_step) {
-  if (!(T == T)) return T;
-  if (T < tT[0]) return tB[0];
-  if (T >= tT[n - 1]) return tB[n - 1];
-  int j = (int)((T - tT[0]) * inv_step);
+// Linear interpolation in the Planck table with np.interp semantics (clamped to
+// the end values outside [T_0, T_last], NaN in -> NaN out), without branches.
+// The interval index comes from the uniform grid spacing; where rounding puts
+// it one interval off (T within ~1e-14 relative of a grid temperature), the
+// neighbouring segment is evaluated at a point it meets within rounding, so the
+// value agrees to ~1e-15. Each LDS entry is (T_j, slope_j, B_j, 0).
+__device__ __forceinline__ double planck_lookup(double T, const double* __restrict__ tab, int n,
+                                                double T_first, double T_last, double inv_step) {
+  const double t = fmin(fmax(T, T_first), T_last);
+  int j = (int)((t - T_first) * inv_step);
   j = j > n - 2 ? n - 2 : j;
-  if (T < tT[j]) {
-    --j;
-  } else if (T >= tT[j + 1]) {
-    ++j;
-  }
-  if (j >= n - 1) return tB[n - 1];
-  const double Tj = tT[j];
-  if (T == Tj) return tB[j];
-  return tS[j] * (T - Tj) + tB[j];
+  const double* e = tab + 4 * j;
+  const double B = e[1] * (t - e[0]) + e[2];
+  return T == T ? B : T;
 }
 
 // Position in a component's frame: offset by X_0 unless the component is
@@ -204,14 +176,14 @@ __device__ __forceinline__ CompFrame comp_frame(ParamPtr P, uint32_t flags, doub
 
 // ---------------------------------------------------------------- densities
 // Cloud, zodipy/number_density.py:47-73:
-// n_0 R^-alpha exp(-beta g^gamma), g = zeta^2/(2 mu) if zeta < mu else zeta - mu/2,
-// evaluated as n_0 exp(-alpha log R - beta g^gamma).
+// n_0 R^-alpha exp_fast(-beta g^gamma), g = zeta^2/(2 mu) if zeta < mu else zeta - mu/2,
+// evaluated as n_0 exp_fast(-alpha log R - beta g^gamma).
 __device__ __forceinline__ double cloud_density(ParamPtr P, const CompFrame& f) {
   const double n0 = P[6], alpha = P[7], beta = P[8], gamma = P[9], mu = P[10];
   const double inv_2mu = P[13], half_mu = P[14];
   const double zeta = fabs(f.zc) * f.inv_r;
   const double g = zeta < mu ? (zeta * zeta) * inv_2mu : zeta - half_mu;
-  return n0 * exp(-alpha * log_fast(f.r) - beta * pow_pos(g, gamma));
+  return n0 * exp_fast(-alpha * log_pos(f.r) - beta * pow_pos(g, gamma));
 }
 
 __device__ __forceinline__ double pow20(double x) {
@@ -223,7 +195,7 @@ __device__ __forceinline__ double pow20(double x) {
 }
 
 // Band, zodipy/number_density.py:76-110 (P[6] holds 3 n_0; the 1/v factor as in
-// the reference): 3 n_0/R exp(-zd^6) (1 + zd^p / v) (1 - exp(-(R/delta_r)^20)),
+// the reference): 3 n_0/R exp_fast(-zd^6) (1 + zd^p / v) (1 - exp_fast(-(R/delta_r)^20)),
 // zd = zeta / delta_zeta.
 __device__ __forceinline__ double band_density(ParamPtr P, uint32_t flags, const CompFrame& f) {
   const double three_n0 = P[6], p = P[8];
@@ -234,18 +206,18 @@ __device__ __forceinline__ double band_density(ParamPtr P, uint32_t flags, const
   const double zd6 = zd4 * zd2;
   const double zdp = (flags & ZODI_CF_P_IS_4) ? zd4 : pow_pos(zd, p);
   const double term3 = 1.0 + zdp * inv_v;
-  const double term4 = 1.0 - exp(-pow20(f.r * inv_dr));
-  return (three_n0 * f.inv_r) * exp(-zd6) * term3 * term4;
+  const double term4 = 1.0 - exp_fast(-pow20(f.r * inv_dr));
+  return (three_n0 * f.inv_r) * exp_fast(-zd6) * term3 * term4;
 }
 
-// Ring, zodipy/number_density.py:113-139: n_0 exp(-(R - R_r)^2/sigma_r^2 - |Z|/sigma_z).
+// Ring, zodipy/number_density.py:113-139: n_0 exp_fast(-(R - R_r)^2/sigma_r^2 - |Z|/sigma_z).
 __device__ __forceinline__ double ring_density(ParamPtr P, const CompFrame& f) {
   const double n0 = P[6], Rr = P[7];
   const double d = f.r - Rr;
-  return n0 * exp(-(d * d) * P[13] - fabs(f.zc) * P[14]);
+  return n0 * exp_fast(-(d * d) * P[13] - fabs(f.zc) * P[14]);
 }
 
-// Feature, zodipy/number_density.py:142-181: n_0 exp(-((R - R_f)^2/sigma_r^2 +
+// Feature, zodipy/number_density.py:142-181: n_0 exp_fast(-((R - R_f)^2/sigma_r^2 +
 // |Z|/sigma_z + dtheta^2/sigma_theta^2)), dtheta wrapped with a floor modulo;
 // `earth_lon` = atan2(E'_y, E'_x) with E' = X_earth - X_0.
 __device__ __forceinline__ double feature_density(ParamPtr P, const CompFrame& f,
@@ -258,7 +230,7 @@ __device__ __forceinline__ double feature_density(ParamPtr P, const CompFrame& f
   double e = (d * d) * P[13];
   e += fabs(f.zc) * P[14];
   e += (dth * dth) * P[15];
-  return n0 * exp(-e);
+  return n0 * exp_fast(-e);
 }
 
 // Fan, zodipy/number_density.py:184-218 (zero outside R <= R_outer).
@@ -268,8 +240,8 @@ __device__ __forceinline__ double fan_density(ParamPtr P, const CompFrame& f) {
   const double beta = asin(f.zc * f.inv_r);
   const double za = fabs(f.zc);
   const double eps = za < Z0 ? 2.0 - za * P[13] : 1.0;
-  const double fv = pow_pos(cos(beta), Q) * exp(-Pp * sin(pow_pos(fabs(beta), eps)));
-  return pow_pos(f.r, -gamma) * fv;
+  const double fv = pow_pos(cos(beta), Q) * exp_fast(-Pp * sin(pow_pos(fabs(beta), eps)));
+  return pow_radius(f.r, -gamma) * fv;
 }
 
 // Comet, zodipy/number_density.py:221-256 (zero outside R_inner <= R <= R_outer).
@@ -279,8 +251,8 @@ __device__ __forceinline__ double comet_density(ParamPtr P, const CompFrame& f) 
   const double beta = asin(f.zc * f.inv_r);
   const double za = fabs(f.zc);
   const double eps = za < Z0 ? 2.0 - za * P[13] : 1.0;
-  const double fv = exp(-Pp * sin(pow_pos(fabs(beta), eps)));
-  return amp * pow_pos(f.r, -gamma) * fv;
+  const double fv = exp_fast(-Pp * sin(pow_pos(fabs(beta), eps)));
+  return amp * pow_radius(f.r, -gamma) * fv;
 }
 
 // Narrow band, zodipy/number_density.py:267-304 (latitude in degrees).
@@ -288,8 +260,8 @@ __device__ __forceinline__ double narrow_band_density(ParamPtr P, const CompFram
   const double bnb = P[6], G = P[7], gamma = P[8], A = P[9], Rin = P[10], Rout = P[11];
   if (!(f.r >= Rin && f.r <= Rout)) return 0.0;
   const double babs = fabs(asin(f.zc * f.inv_r) * kRad2Deg);
-  const double fv = babs < bnb ? exp(G * (babs - bnb)) : 0.0;
-  return A * pow_pos(f.r * P[13], -gamma) * fv;
+  const double fv = babs < bnb ? exp_fast(G * (babs - bnb)) : 0.0;
+  return A * pow_radius(f.r * P[13], -gamma) * fv;
 }
 
 // Broad band, zodipy/number_density.py:307-342 (latitude in degrees).
@@ -299,8 +271,8 @@ __device__ __forceinline__ double broad_band_density(ParamPtr P, const CompFrame
   const double beta = asin(f.zc * f.inv_r) * kRad2Deg;
   const double a = (beta - bbb) * P[13];
   const double c = (beta + bbb) * P[13];
-  const double fv = exp(-0.5 * (a * a)) + exp(-0.5 * (c * c));
-  return A * pow_pos(f.r * P[14], -gamma) * fv;
+  const double fv = exp_fast(-0.5 * (a * a)) + exp_fast(-0.5 * (c * c));
+  return A * pow_radius(f.r * P[14], -gamma) * fv;
 }
 
 __device__ __forceinline__ bool is_feature(int kind) {
@@ -341,7 +313,7 @@ __device__ __forceinline__ double density(int kind, ParamPtr P, uint32_t flags, 
 
 // Scattering phase function at the node (zodipy/scattering.py:11-50):
 // Theta = arccos(-clip(X_los . X_helio / (R_los R_helio), -1, 1)),
-// Phi = N (C1 + C2 Theta + exp(C3 Theta)).
+// Phi = N (C1 + C2 Theta + exp_fast(C3 Theta)).
 __device__ __forceinline__ double phase_function(double R_los, double ux, double uy, double uz,
                                                  double X0, double X1, double X2, double R,
                                                  ParamPtr S) {
@@ -349,7 +321,7 @@ __device__ __forceinline__ double phase_function(double R_los, double ux, double
   double ct = (lx * X0 + ly * X1 + lz * X2) / (R_los * R);
   ct = ct != ct ? ct : (ct < -1.0 ? -1.0 : (ct > 1.0 ? 1.0 : ct));
   const double theta = acos(-ct);
-  return S[ZODI_S_PHASE_NORM] * (S[ZODI_S_C1] + S[ZODI_S_C2] * theta + exp(S[ZODI_S_C3] * theta));
+  return S[ZODI_S_PHASE_NORM] * (S[ZODI_S_C1] + S[ZODI_S_C2] * theta + exp_fast(S[ZODI_S_C3] * theta));
 }
 
 }  // namespace zodi

@@ -146,12 +146,14 @@ __device__ __forceinline__ int64_t find_segment(const int64_t* __restrict__ seg,
   return lo;
 }
 
-__device__ __forceinline__ void load_table(ModelPtr M, double* sT, double* sB, double* sS) {
+// Planck table to LDS as (T_j, slope_j, B_j, 0) entries.
+__device__ __forceinline__ void load_table(ModelPtr M, double* sTab) {
   const int n = M->n_temps;
   for (int i = threadIdx.x; i < n; i += blockDim.x) {
-    sT[i] = M->tab_T[i];
-    sB[i] = M->tab_B[i];
-    sS[i] = M->tab_slope[i];
+    sTab[4 * i] = M->tab_T[i];
+    sTab[4 * i + 1] = M->tab_slope[i];
+    sTab[4 * i + 2] = M->tab_B[i];
+    sTab[4 * i + 3] = 0.0;
   }
   __syncthreads();
 }
@@ -210,9 +212,9 @@ struct Node {
 };
 
 struct Tables {
-  const double* sT;
-  const double* sB;
-  const double* sS;
+  const double* tab;
+  double T_first;
+  double T_last;
   double inv_step;
   int n_t;
 };
@@ -226,8 +228,8 @@ __device__ __forceinline__ Node node_geometry(const Ray& y, double h, double mid
   nd.X2 = s_los * y.uz + y.oz;
   norm_and_inverse(nd.X0 * nd.X0 + nd.X1 * nd.X1 + nd.X2 * nd.X2, nd.R, nd.inv_R);
   // T = T0 R^-delta (zodipy/blackbody.py:16-30), B = interp(T) (zodipy/brightness.py:48)
-  const double T = T0 * pow_pos(nd.R, mdelta);
-  nd.B = planck_lookup(T, tb.sT, tb.sB, tb.sS, tb.n_t, tb.inv_step);
+  const double T = T0 * pow_radius(nd.R, mdelta);
+  nd.B = planck_lookup(T, tb.tab, tb.n_t, tb.T_first, tb.T_last, tb.inv_step);
   nd.phi = 0.0;
   return nd;
 }
@@ -373,13 +375,12 @@ __device__ __forceinline__ void run_layout(Layout<INTEGRAND, Gs...>, const Ctx& 
    ...);
 }
 
-__device__ __forceinline__ Ctx make_ctx(ModelPtr M, const double* sT, const double* sB,
-                                        const double* sS, int sub) {
+__device__ __forceinline__ Ctx make_ctx(ModelPtr M, const double* sTab, int sub) {
   Ctx cx;
   cx.M = M;
-  cx.tb.sT = sT;
-  cx.tb.sB = sB;
-  cx.tb.sS = sS;
+  cx.tb.tab = sTab;
+  cx.tb.T_first = M->tab_T[0];
+  cx.tb.T_last = M->tab_T[M->n_temps - 1];
   cx.tb.inv_step = M->tab_inv_step;
   cx.tb.n_t = M->n_temps;
   cx.K = M->n_nodes;
@@ -391,12 +392,11 @@ __device__ __forceinline__ Ctx make_ctx(ModelPtr M, const double* sT, const doub
 
 template <class LAYOUT, int BLOCK, int L>
 __global__ __launch_bounds__(BLOCK) void forward_layout(FwdArgs a) {
-  __shared__ double sT[ZODI_N_TEMPS];
-  __shared__ double sB[ZODI_N_TEMPS];
-  __shared__ double sS[ZODI_N_TEMPS];
+  __shared__ double sTab[4 * ZODI_N_TEMPS];
   const ModelPtr M = as_const(a.m);
-  load_table(M, sT, sB, sS);
-  const Ctx cx = make_ctx(M, sT, sB, sS, threadIdx.x % L);
+  stage_math_tables();
+  load_table(M, sTab);
+  const Ctx cx = make_ctx(M, sTab, threadIdx.x % L);
   constexpr int RAYS = BLOCK / L;
   const int local = threadIdx.x / L;
   for (int64_t tile = blockIdx.x; tile < a.n_tiles; tile += gridDim.x) {
@@ -473,14 +473,13 @@ __device__ __forceinline__ void generic_groups(const Ctx& cx, const FwdArgs& a, 
 
 template <int BLOCK, int L>
 __global__ __launch_bounds__(BLOCK) void forward_generic(FwdArgs a) {
-  __shared__ double sT[ZODI_N_TEMPS];
-  __shared__ double sB[ZODI_N_TEMPS];
-  __shared__ double sS[ZODI_N_TEMPS];
+  __shared__ double sTab[4 * ZODI_N_TEMPS];
   __shared__ double sAcc[kMaxSlots][512];
   const ModelPtr M = as_const(a.m);
-  load_table(M, sT, sB, sS);
+  stage_math_tables();
+  load_table(M, sTab);
   const int tid = threadIdx.x;
-  const Ctx cx = make_ctx(M, sT, sB, sS, tid % L);
+  const Ctx cx = make_ctx(M, sTab, tid % L);
   constexpr int RAYS = BLOCK / L;
   const int local = tid / L;
   const bool kelsall = M->integrand == ZODI_INTEGRAND_KELSALL;
@@ -522,6 +521,7 @@ __global__ __launch_bounds__(256) void number_density_kernel(const zodi_model* m
                                                              int64_t n, const double* earth,
                                                              double* __restrict__ out) {
   const ModelPtr M = as_const(model);
+  stage_math_tables();
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const double x0 = X[i], x1 = X[n + i], x2 = X[2 * n + i];

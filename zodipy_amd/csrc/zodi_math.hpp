@@ -1,0 +1,96 @@
+// Table-driven FP64 exp and log for the zodiacal kernels (gfx950).
+//
+// exp: Tang's table-driven method (ACM TOMS 15(2):144-157, 1989). x = (64 m + j) ln2/64 + r,
+//   |r| <= ln2/128; exp(x) = 2^m 2^(j/64) (1 + expm1(r)) with expm1(r) from its Taylor
+//   series to r^6 (truncation < 3e-20). About 1.5 ulp; exactly rounded 1 + expm1(r)
+//   for |x| < ln2/128, which keeps 1 - exp(-x) faithful for small x.
+// log: Tang's table-driven method (ACM TOMS 16(4):378-400, 1990). x = 2^e m, m in [1/2, 1),
+//   c_j = 1/2 + j/256 with j the top 7 mantissa bits, r = (m - c_j)/c_j in [0, 1/128);
+//   log(x) = e ln2 + log(c_j) + log1p(r) with log1p from its Taylor series to r^7
+//   (truncation < 2e-18). Absolute error ~1e-16 + 0.5 ulp: the kernels only use log
+//   inside exp(y log x), where the absolute error is what matters.
+//
+// Tables (correctly rounded, gen_math_tables.py) are staged in LDS by every kernel
+// that calls these functions (stage_math_tables); polynomial coefficients sit in
+// constant memory so they are fetched by scalar loads and used as SGPR operands.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "zodi_math_tables.inc"
+
+namespace zodi {
+
+__constant__ double kMathTablesConst[64 + 128 + 128] = {ZODI_EXP2_TABLE, ZODI_LOG_INV_TABLE,
+                                                        ZODI_LOG_C_TABLE};
+// Taylor coefficients: expm1 r^6..r^3 (1/720, 1/120, 1/24, 1/6); log1p r^7..r^3.
+__constant__ double kExpPoly[4] = {1.0 / 720.0, 1.0 / 120.0, 1.0 / 24.0, 1.0 / 6.0};
+__constant__ double kLogPoly[5] = {1.0 / 7.0, -1.0 / 6.0, 1.0 / 5.0, -1.0 / 4.0, 1.0 / 3.0};
+
+__shared__ double g_exp2[64];
+__shared__ double g_log_inv[128];
+__shared__ double g_log_c[128];
+
+// Copy the tables to LDS; every thread of the block must call this before use.
+__device__ __forceinline__ void stage_math_tables() {
+  for (int i = threadIdx.x; i < 64; i += blockDim.x) g_exp2[i] = kMathTablesConst[i];
+  for (int i = threadIdx.x; i < 128; i += blockDim.x) {
+    g_log_inv[i] = kMathTablesConst[64 + i];
+    g_log_c[i] = kMathTablesConst[192 + i];
+  }
+  __syncthreads();
+}
+
+__device__ __forceinline__ double exp_fast(double x) {
+  constexpr double kInvL = 92.33248261689366;           // 64 / ln 2
+  constexpr double kLHi = 0x1.62e42fee00000p-7;         // ln2/64, 32 significant bits
+  constexpr double kLLo = 0x1.a39ef35793c76p-39;        // ln2/64 - kLHi
+  // Every exponent in the model is below ~50 (the positive ones are small
+  // temperature and power-law exponents); the lower clamp keeps the integer
+  // conversion in range and exp(-746) rounds to 0 (also for -inf). NaN passes.
+  const double xc = x < -746.0 ? -746.0 : x;
+  const double kd = __builtin_rint(xc * kInvL);
+  double r = fma(-kd, kLHi, xc);
+  r = fma(-kd, kLLo, r);
+  const int k = (int)kd;
+  const int j = k & 63;
+  const int m = k >> 6;
+  double p = fma(kExpPoly[0], r, kExpPoly[1]);
+  p = fma(p, r, kExpPoly[2]);
+  p = fma(p, r, kExpPoly[3]);
+  p = fma(p, r, 0.5);
+  const double q = fma(p * r, r, r);  // expm1(r)
+  const double t = g_exp2[j];
+  return __builtin_amdgcn_ldexp(fma(t, q, t), m);
+}
+
+// log for positive finite (or NaN) arguments: no special-case handling.
+__device__ __forceinline__ double log_pos(double x) {
+  constexpr double kLn2Hi = 0x1.62e42fee00000p-1;  // ln 2, 32 significant bits
+  constexpr double kLn2Lo = 0x1.a39ef35793c76p-33;  // ln 2 - kLn2Hi
+  const int e = __builtin_amdgcn_frexp_exp(x);
+  const double m = __builtin_amdgcn_frexp_mant(x);  // [1/2, 1) for finite x != 0
+  const uint32_t hi = (uint32_t)(__builtin_bit_cast(uint64_t, m) >> 32);
+  const int j = (int)((hi >> 13) & 127u);
+  const double c = fma((double)j, 1.0 / 256.0, 0.5);
+  const double r = (m - c) * g_log_inv[j];  // m - c is exact
+  double p = fma(kLogPoly[0], r, kLogPoly[1]);
+  p = fma(p, r, kLogPoly[2]);
+  p = fma(p, r, kLogPoly[3]);
+  p = fma(p, r, kLogPoly[4]);
+  p = fma(p, r, -0.5);
+  const double l1p = fma(p * r, r, r);  // log1p(r)
+  const double de = (double)e;
+  return fma(de, kLn2Hi, g_log_c[j] + fma(de, kLn2Lo, l1p));
+}
+
+// log for any argument: 0 -> -inf, negative or NaN -> NaN, +inf -> +inf.
+__device__ __forceinline__ double log_fast(double x) {
+  const double res = log_pos(x);
+  if (x > 0.0 && x < __builtin_inf()) return res;
+  if (x == 0.0) return -__builtin_inf();
+  return x == __builtin_inf() ? x : __builtin_nan("");
+}
+
+}  // namespace zodi
