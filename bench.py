@@ -10,8 +10,9 @@ Also reported (not the headline value):
 * plain: the per-observer drop-in loop (host pointings -> H2D, one launch,
   D2H, sync, per observer) over the same observers;
 * batched_e2e: the batch including the H2D of all pointings and D2H of the output;
-* fwd_grad_over_fwd: forward + adjoint time over forward time (DIRBE 25 um,
-  nside=128 chi^2 gradient) when the adjoint is available;
+* fwd_grad_over_fwd: fused forward + adjoint (chi^2 gradient) time over forward
+  time, DIRBE 25 um at nside=128; ``grad`` adds the batched fused gradient of
+  planck18 at 857 GHz over 1000 observers;
 * cpu_baseline: the NumPy oracle with the reference's loop structure and its
   fork-pool split over host cores, one pool per observer call as the reference
   creates it, on a bounded prefix of the same observers.
@@ -135,6 +136,95 @@ def cpu_baseline(u, seg, obs, budget_s: float) -> dict:
                    f"workload, NumPy oracle, fork pool of {nproc} processes per observer call; "
                    f"{dt:.1f} s wall on {cpu_model} ({platform.machine()})"),
     }
+
+
+def _event_time(fn, reps: int) -> float:
+    """Average ms of ``fn`` over ``reps`` calls, HIP events on the current stream."""
+    import torch
+
+    stream = torch.cuda.current_stream()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record(stream)
+    for _ in range(reps):
+        fn()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / reps
+
+
+def grad_cost(dev, reps: int = 10) -> dict:
+    """Forward + chi^2 gradient over forward, and the batched fused gradient.
+
+    (a) DIRBE 25 um, HEALPix nside=128 pixel centres, observer and Earth at
+        orbital phase 0.3 rad: target = model at theta with every parameter
+        perturbed by seeded N(0, 1%) relative noise plus N(0, 0.01 MJy/sr);
+        loss chi^2 (inverse variance 1e4). Fused forward + adjoint (one launch)
+        against the forward launch alone, both at theta0.
+    (b) planck18 at 857 GHz with a 30-frequency top-hat bandpass of +-10%,
+        1000 observers on a 1 AU orbit with seeded 5000-40000 pointings each:
+        fused forward + adjoint over the whole batch against the batched forward.
+    """
+    import torch
+
+    from zodipy_amd import Model, healpix
+    from zodipy_amd import units as zu
+    from zodipy_amd.fitting import LineOfSightProblem
+
+    out = {}
+    model = Model(25 * zu.micron, name="dirbe", device=dev)
+    v = healpix.pix2vec(128)
+    ph = 0.3
+    obs = np.array([np.cos(ph), np.sin(ph), 0.0])
+    prob = LineOfSightProblem(model, v, obs, obs)
+    theta0 = prob.space.theta0
+    rng = np.random.default_rng(SEED + 4)
+    theta_true = theta0 * (1.0 + 0.01 * rng.normal(size=theta0.size))
+    noise = torch.as_tensor(0.01 * rng.normal(size=v.shape[1]), device=dev)
+    data = prob.forward(theta_true) + noise
+    inv_var = 1e4
+    prob.forward(theta0)
+    prob.chi2_and_grad(theta0, data, inv_var)
+    t_fwd = _event_time(lambda: prob.forward(theta0), reps)
+    t_fg = _event_time(lambda: prob.chi2_and_grad(theta0, data, inv_var), reps)
+    c2, g = prob.chi2_and_grad(theta0, data, inv_var)
+    out["chi2_dirbe25_nside128"] = {
+        "n_rays": int(v.shape[1]), "n_params": prob.n_params,
+        "fwd_ms": round(t_fwd, 4), "fwd_grad_ms": round(t_fg, 4),
+        "ratio": round(t_fg / t_fwd, 4), "chi2_at_theta0": float(c2),
+        "grad_norm": float(torch.linalg.vector_norm(g)),
+    }
+    log(f"chi2 gradient (DIRBE 25 um, nside 128, {prob.n_params} params): fwd {t_fwd:.3f} ms, "
+        f"fwd+grad {t_fg:.3f} ms, ratio {t_fg / t_fwd:.3f}")
+
+    nu = np.linspace(857 * 0.9, 857 * 1.1, 30) * zu.GHz
+    # the band edge (942.7 GHz) lies beyond planck18's spectral table: extrapolate
+    pmodel = Model(nu, weights=np.ones(30), name="planck18", extrapolate=True, device=dev)
+    rng = np.random.default_rng(SEED + 5)
+    n_obs = 1000
+    counts = rng.integers(5000, 40000, n_obs, endpoint=True)
+    seg = np.zeros(n_obs + 1, dtype=np.int64)
+    np.cumsum(counts, out=seg[1:])
+    phases = rng.uniform(0.0, 2.0 * np.pi, n_obs)
+    pobs = np.stack([np.cos(phases), np.sin(phases), np.zeros(n_obs)])
+    u_b = make_directions(int(seg[-1]), dev, SEED + 5).contiguous()
+    bprob = LineOfSightProblem(pmodel, u_b, pobs, pobs, seg_offsets=seg)
+    th = bprob.space.theta0
+    bdata = bprob.forward(th) * 1.01
+    bprob.chi2_and_grad(th, bdata)
+    bt_fwd = _event_time(lambda: bprob.forward(th), max(2, reps // 2))
+    bt_fg = _event_time(lambda: bprob.chi2_and_grad(th, bdata), max(2, reps // 2))
+    out["batched_planck18_857GHz"] = {
+        "n_rays": int(seg[-1]), "n_observers": n_obs, "n_params": bprob.n_params,
+        "fwd_ms": round(bt_fwd, 3), "fwd_grad_ms": round(bt_fg, 3),
+        "ratio": round(bt_fg / bt_fwd, 4),
+        "fwd_grad_ray_node_samples_per_s": int(seg[-1]) * N_NODES / (bt_fg / 1e3),
+    }
+    log(f"batched fused gradient (planck18 857 GHz, {int(seg[-1])} rays): fwd {bt_fwd:.2f} ms, "
+        f"fwd+grad {bt_fg:.2f} ms, ratio {bt_fg / bt_fwd:.3f}")
+    out["ratio"] = out["chi2_dirbe25_nside128"]["ratio"]
+    return out
 
 
 def main() -> int:
@@ -279,13 +369,7 @@ def main() -> int:
 
     grad_ratio = None
     if not args.no_grad:
-        try:
-            from zodipy_amd.fitting import fwd_grad_ratio
-
-            grad_ratio = fwd_grad_ratio(dev)
-            log(f"fwd+grad / fwd: {grad_ratio['ratio']:.3f}")
-        except ImportError:
-            grad_ratio = None
+        grad_ratio = grad_cost(dev)
 
     line = {
         "metric": "ray-node samples/sec (FP64) batched vs plain, and fwd+grad/fwd time ratio",

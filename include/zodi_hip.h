@@ -37,7 +37,8 @@ extern "C" {
 #define ZODI_MAX_GROUPS 16
 #define ZODI_MAX_NODES 512
 #define ZODI_N_TEMPS 100
-#define ZODI_COMP_SLOTS 24
+#define ZODI_COMP_SLOTS 32
+#define ZODI_MAX_PARAMS 256
 #define ZODI_SHARED_SLOTS 8
 
 typedef int32_t zodi_status;
@@ -80,11 +81,13 @@ enum { ZODI_P_X0 = 0, ZODI_P_Y0 = 1, ZODI_P_Z0 = 2,
        ZODI_P_K0 = 6,   /* first kind-specific parameter slot (6..12) */
        ZODI_P_D0 = 13,  /* first kind-specific derived slot (13..19) */
        ZODI_P_EMISSIVITY = 20, ZODI_P_ALBEDO = 21, ZODI_P_ONE_MINUS_ALBEDO = 22,
-       ZODI_P_SCALE = 23 /* (1 - albedo) * emissivity */ };
+       ZODI_P_SCALE = 23, /* (1 - albedo) * emissivity */
+       ZODI_P_SIN_I = 24, ZODI_P_COS_I = 25, ZODI_P_SIN_OMEGA = 26, ZODI_P_COS_OMEGA = 27 };
 
 /* Shared parameter slots. */
 enum { ZODI_S_C1 = 0, ZODI_S_C2 = 1, ZODI_S_C3 = 2, ZODI_S_SOLAR = 3,
-       ZODI_S_PHASE_NORM = 4, ZODI_S_CALIBRATION = 5 };
+       ZODI_S_PHASE_NORM = 4, ZODI_S_CALIBRATION = 5,
+       ZODI_S_DD_DC3 = 6 /* dD/dC3 of the phase normalisation N = 1/(2 pi D) */ };
 
 /*
  * Device model block: uploaded once per model construction or parameter update.
@@ -108,8 +111,9 @@ enum { ZODI_S_C1 = 0, ZODI_S_C2 = 1, ZODI_S_C3 = 2, ZODI_S_SOLAR = 3,
  * Derived slots, from ZODI_P_D0 (the kernels multiply instead of dividing):
  *   CLOUD        1/(2 mu), mu/2
  *   BAND         1/delta_zeta_rad, 1/v, 1/delta_r
- *   RING(_RRM)   1/sigma_r^2, 1/sigma_z
- *   FEATURE(_RRM) 1/sigma_r^2, 1/sigma_z, 1/sigma_theta_rad^2
+ *   RING(_RRM)   1/sigma_r^2, 1/sigma_z, 1/sigma_r
+ *   FEATURE(_RRM) 1/sigma_r^2, 1/sigma_z, 1/sigma_theta_rad^2, 1/sigma_r,
+ *                1/sigma_theta_rad
  *   FAN, COMET   1/Z_0
  *   NARROW_BAND  1/R_outer
  *   BROAD_BAND   1/sigma_bb, 1/R_outer
@@ -127,6 +131,14 @@ typedef struct zodi_model {
   uint32_t comp_flags[ZODI_MAX_COMPS];
   int32_t group_first[ZODI_MAX_GROUPS];
   int32_t group_count[ZODI_MAX_GROUPS];
+  /* Parameter-vector positions for zodi_adjoint (see there). */
+  int32_t grad_comp[ZODI_MAX_COMPS];    /* first of the component's init fields */
+  int32_t grad_spec_a[ZODI_MAX_COMPS];  /* Kelsall: emissivity; RRM: T_0 of the component */
+  int32_t grad_spec_b[ZODI_MAX_COMPS];  /* Kelsall: albedo; RRM: delta of the component */
+  int32_t grad_shared[8];               /* Kelsall: T_0, delta, solar irradiance, C1, C2, C3;
+                                           RRM: calibration */
+  int32_t n_params;                     /* length of the parameter vector */
+  int32_t reserved2;
   double group_r_in[ZODI_MAX_GROUPS];   /* start cutoff radius [AU] */
   double group_r_out[ZODI_MAX_GROUPS];  /* stop cutoff radius [AU] */
   double group_T0[ZODI_MAX_GROUPS];     /* T(R) = T0 R^-delta */
@@ -209,6 +221,46 @@ zodi_status zodi_line_of_sight(const zodi_model* model_host, const zodi_model* m
 zodi_status zodi_number_density(const zodi_model* model_host, const zodi_model* model_dev,
                                 const double* X, int64_t n_points,
                                 const double* earth, double* out, void* stream);
+
+/* Adjoint modes. */
+enum {
+  ZODI_ADJ_VJP_SUM = 0,    /* weights [n_rays] against the component sum */
+  ZODI_ADJ_VJP_COMPS = 1,  /* weights [n_comps][n_rays] against each component */
+  ZODI_ADJ_CHI2 = 2        /* evaluates I, chi2 = sum (I - target)^2 inv_var, and its
+                              gradient (weights 2 (I - target) inv_var) */
+};
+
+/*
+ * zodi_adjoint — gradient of sum_r sum_c weights_c,r I_c,r (or of chi2) with respect
+ * to the parameter vector whose layout the model block's grad_* fields give.
+ * No reference counterpart: the reference has no derivatives; its parameter
+ * dictionary (zodipy/zodiacal_light_model.py:38-55, zodipy/model.py:302-333) fixes
+ * the order of the component entries.
+ *
+ * Rays are addressed as in zodi_forward. grad is [n_params + 1] in device memory;
+ * entry n_params is chi2 in ZODI_ADJ_CHI2 mode and 0 otherwise. inv_var is
+ * [n_rays] or NULL (then inv_var_scalar applies to every ray). In ZODI_ADJ_CHI2
+ * mode the forward kernel first writes I [n_rays] to model_out, or to the end of
+ * the workspace when model_out is NULL. workspace holds zodi_adjoint_workspace()
+ * bytes of device memory for the same n_rays and launch. launch may be NULL;
+ * otherwise block must be 0 or 256, lanes_per_ray 0 (chosen from n_rays), 1, 2 or
+ * 4 (lanes sharing one ray's nodes), blocks_per_cu 0..64 (0 = 2), and variant the
+ * components evaluated per pass over a group's nodes: 0 (default, 1), 1 or 2.
+ * Supported layouts: the Kelsall registry models (cloud and three bands,
+ * optionally followed by the ring and the feature); others return
+ * ZODI_ERR_MODEL. The reduction order is fixed: equal inputs give equal bits.
+ */
+int64_t zodi_adjoint_workspace(const zodi_model* model_host, int64_t n_rays,
+                               const zodi_launch* launch);
+
+zodi_status zodi_adjoint(const zodi_model* model_host, const zodi_model* model_dev,
+                         const double* u, int64_t n_rays, const int64_t* seg_offsets,
+                         int32_t n_seg, const double* obs, const double* earth,
+                         int32_t obs_mode, const uint32_t* beyond, int32_t mode,
+                         const double* weights, const double* target, const double* inv_var,
+                         double inv_var_scalar, double* model_out, double* grad,
+                         double* workspace, int64_t workspace_bytes, const zodi_launch* launch,
+                         void* stream);
 
 #ifdef __cplusplus
 }

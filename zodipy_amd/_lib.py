@@ -18,6 +18,7 @@ LIB_PATH = Path(__file__).resolve().parent / "libzodi_hip.so"
 
 OBS_PER_SEGMENT, OBS_PER_RAY = 0, 1
 OUT_COMPS, OUT_SUM = 0, 1
+ADJ_VJP_SUM, ADJ_VJP_COMPS, ADJ_CHI2 = 0, 1, 2
 
 EXPORTED_SYMBOLS = (
     "zodi_abi_version",
@@ -26,6 +27,8 @@ EXPORTED_SYMBOLS = (
     "zodi_forward",
     "zodi_line_of_sight",
     "zodi_number_density",
+    "zodi_adjoint_workspace",
+    "zodi_adjoint",
 )
 
 
@@ -61,6 +64,11 @@ def _bind(lib: ct.CDLL) -> None:
     lib.zodi_line_of_sight.argtypes = [_p, _p, _p, _i64, _p, _i32, _p, _i32, _p, _p, _p, _p]
     lib.zodi_number_density.restype = ct.c_int32
     lib.zodi_number_density.argtypes = [_p, _p, _p, _i64, _p, _p, _p]
+    lib.zodi_adjoint_workspace.restype = ct.c_int64
+    lib.zodi_adjoint_workspace.argtypes = [_p, _i64, ct.POINTER(Launch)]
+    lib.zodi_adjoint.restype = ct.c_int32
+    lib.zodi_adjoint.argtypes = [_p, _p, _p, _i64, _p, _i32, _p, _p, _i32, _p, _i32, _p, _p,
+                                 _p, ct.c_double, _p, _p, _p, _i64, ct.POINTER(Launch), _p]
 
 
 def load() -> ct.CDLL:

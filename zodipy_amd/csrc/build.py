@@ -17,8 +17,14 @@ PKG = CSRC.parent
 ROOT = PKG.parent
 INCLUDE = ROOT / "include"
 LIB = PKG / "libzodi_hip.so"
-SOURCES = [CSRC / "zodi_kernels.hip"]
-HEADERS = [CSRC / "zodi_device.hpp", INCLUDE / "zodi_hip.h"]
+SOURCES = [CSRC / "zodi_forward.hip", CSRC / "zodi_adjoint.hip"]
+HEADERS = [
+    CSRC / "zodi_common.hpp",
+    CSRC / "zodi_device.hpp",
+    CSRC / "zodi_grad.hpp",
+    CSRC / "zodi_math.hpp",
+    INCLUDE / "zodi_hip.h",
+]
 
 ARCH = os.environ.get("ZODI_OFFLOAD_ARCH", "gfx950")
 

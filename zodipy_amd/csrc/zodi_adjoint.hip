@@ -1,0 +1,550 @@
+// Adjoint (vector-Jacobian product) of the line-of-sight integral, gfx950 FP64.
+//
+// For weights g (per ray, or per component and ray) the kernels form
+//   grad_p = sum_rays sum_c g_c,r dI_c,r / dtheta_p
+// for every model parameter theta_p, recomputing the integrand (no stored node
+// values). The integral of component c along ray r is I = h sum_k w_k f(s_k)
+// with h = (stop - start)/2 fixed by the cutoffs, so dI/dtheta = h sum_k w_k
+// df/dtheta. In the chi^2 mode the forward kernel first writes the model values
+// I; the adjoint kernel then weights each ray by g = 2 (I - d) / sigma^2 and
+// also sums chi^2.
+//
+// One wave owns a tile of 64 rays, one ray per lane. Per chunk of components
+// of a group (components sharing a range; the chunk size is a compile-time
+// choice that trades recomputed node geometry for register pressure), each
+// node's geometry, B(T), dB/dT and the phase function with its C1..C3 partials
+// are formed once; every component of the chunk then adds its contributions to
+// register accumulators sized at compile time (layout templates). After a
+// chunk's node loop the wave sums each accumulator over its lanes with an xor
+// butterfly (every lane obtains the same bits) and lane 0 adds the sums to the
+// wave's gradient row in LDS. At the end the block
+// adds its wave rows in wave order into one partial vector and a second kernel
+// adds the partials in block order: the result is reproducible bit for bit.
+//
+// The parameter vector follows zodipy/zodiacal_light_model.py:38-55 (component
+// init fields in declaration order) followed by the effective spectral
+// parameters; offsets come from the model block (grad_* fields).
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <tuple>
+#include <utility>
+
+#include "zodi_common.hpp"
+#include "zodi_grad.hpp"
+
+namespace zodi {
+
+constexpr int kWave = 64;
+constexpr int kAdjWaves = 4;
+constexpr int kAdjBlock = kWave * kAdjWaves;
+
+struct AdjArgs {
+  FwdArgs f;
+  const double* weights;   // VJP_SUM: [n_rays]; VJP_COMPS: [n_comps][n_rays]
+  const double* target;    // CHI2: data [n_rays]
+  const double* inv_var;   // CHI2: 1/sigma^2 per ray, or null for inv_var_scalar
+  double inv_var_scalar;
+  const double* model_vals;  // CHI2: model values I [n_rays] (forward kernel output)
+  double* partials;          // [gridDim.x][n_params + 1]
+  int32_t n_params;
+  int32_t mode;              // ZODI_ADJ_*
+};
+
+// Sum over the 64 lanes of a wave; a + b == b + a in IEEE arithmetic, so every
+// lane ends with identical bits.
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
+  return v;
+}
+
+// B(T) of the Planck table and its slope dB/dT (0 where the table clamps).
+__device__ __forceinline__ double planck_lookup_slope(double T, const double* __restrict__ tab,
+                                                      int n, double T_first, double T_last,
+                                                      double inv_step, double& dBdT) {
+  const double t = fmin(fmax(T, T_first), T_last);
+  int j = (int)((t - T_first) * inv_step);
+  j = j > n - 2 ? n - 2 : j;
+  const double* e = tab + 4 * j;
+  dBdT = (T > T_first && T < T_last) ? e[1] : 0.0;
+  const double B = e[1] * (t - e[0]) + e[2];
+  return T == T ? B : T;
+}
+
+template <int NP>
+struct CompAcc {
+  double v[5 + NP + 2];  // geometry (5), kind parameters (NP), two spectral
+};
+
+// Components J (consecutive, from S) of group g, whose first component is `first`.
+// Lane `sub` of the SPLIT lanes sharing the ray takes nodes sub, sub + SPLIT, ...
+template <int INTEGRAND, int SPLIT, int S, int... Ks, int... J>
+__device__ __forceinline__ void adjoint_group(Grp<Ks...>, std::integer_sequence<int, J...>,
+                                              const Ctx& cx, const AdjArgs& a, const Ray& y,
+                                              int g, int first, double gscale, double* row,
+                                              int lane, int sub) {
+  constexpr int C = sizeof...(Ks);
+  constexpr int N = sizeof...(J);
+  constexpr int kinds[C] = {Ks...};
+  const ModelPtr M = cx.M;
+  double h, mid;
+  group_range(M, g, y, h, mid);
+  const double T0 = M->group_T0[g];
+  const double mdelta = -M->group_delta[g];
+  const double inv_T0 = 1.0 / T0;
+
+  double W[N];  // h times the weight of each component
+  if (a.mode == ZODI_ADJ_VJP_COMPS) {
+    ((W[J - S] = y.active ? h * a.weights[(int64_t)(first + J) * a.f.n_rays + y.r] : 0.0), ...);
+  } else {
+    ((W[J - S] = h * gscale), ...);
+  }
+  double elon[N];
+  ((elon[J - S] = is_feature(kinds[J]) ? atan2(y.ey - M->comp[first + J][ZODI_P_Y0],
+                                               y.ex - M->comp[first + J][ZODI_P_X0])
+                                       : 0.0),
+   ...);
+  EarthTerm et[N];
+  ((et[J - S] = is_feature(kinds[J]) ? earth_term(M->comp[first + J], y.ex, y.ey)
+                                     : EarthTerm{0.0, 0.0}),
+   ...);
+  const double solar = cx.solar;
+  // Phi enters d/d(albedo) when F != 0, and d/dF, d/dC_i when an albedo is non-zero.
+  const bool need_phase =
+      INTEGRAND == ZODI_INTEGRAND_KELSALL && (solar != 0.0 || (M->flags & 1) != 0);
+
+  std::tuple<CompAcc<kind_nparams<kinds[J]>()>...> acc{};
+  double sh[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};  // Kelsall: T0, delta, F, C1..C3; RRM: cal
+
+  const int K = y.active ? cx.K : 0;
+  for (int k = sub; k < K; k += SPLIT) {
+    const ModelPtr Mk = opaque(M);
+    const double xk = Mk->gl_x[k];
+    const double wk = Mk->gl_w[k];
+    const double s_los = h * xk + mid;
+    Node nd;
+    nd.X0 = s_los * y.ux + y.ox;
+    nd.X1 = s_los * y.uy + y.oy;
+    nd.X2 = s_los * y.uz + y.oz;
+    norm_and_inverse(nd.X0 * nd.X0 + nd.X1 * nd.X1 + nd.X2 * nd.X2, nd.R, nd.inv_R);
+    const double logR = log_pos(nd.R);
+    const double T = T0 * exp_fast(mdelta * logR);
+    double dBdT;
+    nd.B = planck_lookup_slope(T, cx.tb.tab, cx.tb.n_t, cx.tb.T_first, cx.tb.T_last,
+                               cx.tb.inv_step, dBdT);
+    const double dT_dT0 = T * inv_T0;
+    const double dT_ddelta = -T * logR;
+    PhaseGrad pg = {0.0, 0.0, 0.0, 0.0};
+    if (need_phase) {
+      pg = phase_grad(y.ux, y.uy, y.uz, nd.X0, nd.X1, nd.X2, nd.inv_R, Mk->shared);
+    }
+    const double inv_R2 = nd.inv_R * nd.inv_R;
+    (
+        [&] {
+          constexpr int KIND = kinds[J];
+          constexpr int NP = kind_nparams<KIND>();
+          const ParamPtr P = Mk->comp[first + J];
+          const uint32_t flags = Mk->comp_flags[first + J];
+          const CompFrame f = comp_frame(P, flags, nd.X0, nd.X1, nd.X2, nd.R, nd.inv_R);
+          const DensGrad<NP> dg = density_grad<KIND>(P, flags, f, elon[J - S], logR);
+          double geo[5];
+          chain_geometry<is_feature(KIND)>(P, f, dg.dR, dg.dZ, dg.dth, et[J - S], geo);
+          auto& A = std::get<J - S>(acc).v;
+          const double wn = wk * W[J - S];
+          if constexpr (INTEGRAND == ZODI_INTEGRAND_KELSALL) {
+            // f = [(1 - a) eps B + a F Phi / R^2] n   (zodipy/brightness.py:21-56)
+            const double alb = P[ZODI_P_ALBEDO], eps = P[ZODI_P_EMISSIVITY];
+            const double oma = P[ZODI_P_ONE_MINUS_ALBEDO];
+            const double scat_unit = (solar * inv_R2) * pg.phi;  // F Phi / R^2
+            double e = oma * (eps * nd.B);
+            if (flags & ZODI_CF_SCATTER) e += alb * scat_unit;
+            const double we = wn * e;
+#pragma unroll
+            for (int q = 0; q < 5; ++q) A[q] += we * geo[q];
+#pragma unroll
+            for (int q = 0; q < NP; ++q) A[5 + q] += we * dg.dp[q];
+            const double wnn = wn * dg.n;
+            A[5 + NP] += wnn * (oma * nd.B);                  // emissivity
+            A[5 + NP + 1] += wnn * (scat_unit - eps * nd.B);  // albedo
+            const double wt = wnn * (oma * eps) * dBdT;
+            sh[0] += wt * dT_dT0;
+            sh[1] += wt * dT_ddelta;
+            const double was = wnn * alb * inv_R2;
+            sh[2] += was * pg.phi;
+            const double wasf = was * solar;
+            sh[3] += wasf * pg.d1;
+            sh[4] += wasf * pg.d2;
+            sh[5] += wasf * pg.d3;
+          } else {
+            // f = calibration B(T_c) n   (zodipy/brightness.py:59-83)
+            const double wc = wn * cx.calibration;
+            const double wcb = wc * nd.B;
+#pragma unroll
+            for (int q = 0; q < 5; ++q) A[q] += wcb * geo[q];
+#pragma unroll
+            for (int q = 0; q < NP; ++q) A[5 + q] += wcb * dg.dp[q];
+            const double wt = wc * dg.n * dBdT;
+            A[5 + NP] += wt * dT_dT0;         // T_0 of this component
+            A[5 + NP + 1] += wt * dT_ddelta;  // delta of this component
+            sh[0] += wn * nd.B * dg.n;        // calibration
+          }
+        }(),
+        ...);
+  }
+
+  // Wave sums into the wave's gradient row (lane 0 writes).
+  (
+      [&] {
+        constexpr int NP = kind_nparams<kinds[J]>();
+        auto& A = std::get<J - S>(acc).v;
+#pragma unroll
+        for (int q = 0; q < 5 + NP + 2; ++q) A[q] = wave_sum(A[q]);
+        if (lane == 0) {
+          const int c = first + J;
+          const int off = M->grad_comp[c];
+#pragma unroll
+          for (int q = 0; q < 5 + NP; ++q) row[off + q] += A[q];
+          row[M->grad_spec_a[c]] += A[5 + NP];
+          row[M->grad_spec_b[c]] += A[5 + NP + 1];
+        }
+      }(),
+      ...);
+  constexpr int NSH = INTEGRAND == ZODI_INTEGRAND_KELSALL ? 6 : 1;
+#pragma unroll
+  for (int q = 0; q < NSH; ++q) sh[q] = wave_sum(sh[q]);
+  if (lane == 0) {
+#pragma unroll
+    for (int q = 0; q < NSH; ++q) row[M->grad_shared[q]] += sh[q];
+  }
+}
+
+template <int S, int... I>
+constexpr std::integer_sequence<int, (S + I)...> shift_seq(std::integer_sequence<int, I...>) {
+  return {};
+}
+
+// Group g in chunks of at most CHUNK components, from component S of the group.
+template <int INTEGRAND, int SPLIT, int CHUNK, int S, int... Ks>
+__device__ __forceinline__ void adjoint_chunks(Grp<Ks...> grp, const Ctx& cx, const AdjArgs& a,
+                                               const Ray& y, int g, int first, double gscale,
+                                               double* row, int lane, int sub) {
+  constexpr int C = sizeof...(Ks);
+  if constexpr (S < C) {
+    constexpr int N = (C - S) < CHUNK ? (C - S) : CHUNK;
+    adjoint_group<INTEGRAND, SPLIT, S>(grp, shift_seq<S>(std::make_integer_sequence<int, N>{}),
+                                       cx, a, y, g, first, gscale, row, lane, sub);
+    adjoint_chunks<INTEGRAND, SPLIT, CHUNK, S + N>(grp, cx, a, y, g, first, gscale, row, lane,
+                                                   sub);
+  }
+}
+
+template <int SPLIT, int CHUNK, int INTEGRAND, class... Gs>
+__device__ __forceinline__ void run_adjoint(Layout<INTEGRAND, Gs...>, const Ctx& cx,
+                                            const AdjArgs& a, const Ray& y, double gscale,
+                                            double* row, int lane, int sub) {
+  int g = 0;
+  int first = 0;
+  ((adjoint_chunks<INTEGRAND, SPLIT, CHUNK, 0>(Gs{}, cx, a, y, g, first, gscale, row, lane, sub),
+    ++g, first += Gs::size),
+   ...);
+}
+
+// One wave per tile of 64 / SPLIT rays; SPLIT lanes share a ray's nodes.
+template <class LAYOUT, int CHUNK, int SPLIT>
+__global__ __launch_bounds__(kAdjBlock) void adjoint_layout(AdjArgs a) {
+  constexpr int RAYS = kWave / SPLIT;
+  extern __shared__ double sRows[];  // kAdjWaves rows of n_params + 1
+  __shared__ double sTab[4 * ZODI_N_TEMPS];
+  const ModelPtr M = as_const(a.f.m);
+  const int tid = threadIdx.x;
+  const int wave = tid / kWave;
+  const int lane = tid % kWave;
+  const int sub = lane % SPLIT;
+  const int np1 = a.n_params + 1;
+  for (int i = tid; i < np1 * kAdjWaves; i += kAdjBlock) sRows[i] = 0.0;
+  stage_math_tables();
+  load_table(M, sTab);  // ends with a barrier
+  const Ctx cx = make_ctx(M, sTab, 0);
+  double* row = sRows + wave * np1;
+
+  const int64_t stride = (int64_t)gridDim.x * kAdjWaves;
+  for (int64_t tile = (int64_t)blockIdx.x * kAdjWaves + wave; tile < a.f.n_tiles;
+       tile += stride) {
+    const Ray y = load_ray<RAYS>(a.f, tile, lane / SPLIT);
+    double gscale = 0.0;
+    if (a.mode == ZODI_ADJ_CHI2) {
+      double chi = 0.0;
+      if (y.active) {
+        const double I = a.model_vals[y.r];
+        const double iv = a.inv_var != nullptr ? a.inv_var[y.r] : a.inv_var_scalar;
+        const double res = I - a.target[y.r];
+        chi = sub == 0 ? res * res * iv : 0.0;
+        gscale = 2.0 * res * iv;
+      }
+      chi = wave_sum(chi);
+      if (lane == 0) row[a.n_params] += chi;
+    } else if (a.mode == ZODI_ADJ_VJP_SUM) {
+      gscale = y.active ? a.weights[y.r] : 0.0;
+    }
+    run_adjoint<SPLIT, CHUNK>(LAYOUT{}, cx, a, y, gscale, row, lane, sub);
+  }
+  __syncthreads();
+  for (int p = tid; p < np1; p += kAdjBlock) {
+    double s = sRows[p];
+#pragma unroll
+    for (int w = 1; w < kAdjWaves; ++w) s += sRows[w * np1 + p];
+    a.partials[(int64_t)blockIdx.x * np1 + p] = s;
+  }
+}
+
+// Sum of the block partials in block order.
+__global__ __launch_bounds__(256) void adjoint_reduce(const double* __restrict__ partials,
+                                                      int n_blocks, int np1,
+                                                      double* __restrict__ grad) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= np1) return;
+  double s = 0.0;
+  for (int b = 0; b < n_blocks; ++b) s += partials[(int64_t)b * np1 + p];
+  grad[p] = s;
+}
+
+namespace {
+
+using AdjKernel = void (*)(AdjArgs);
+
+template <class LAYOUT, int CHUNK>
+AdjKernel pick_adj_split(int split) {
+  switch (split) {
+    case 1:
+      return adjoint_layout<LAYOUT, CHUNK, 1>;
+    case 2:
+      return adjoint_layout<LAYOUT, CHUNK, 2>;
+    case 4:
+      return adjoint_layout<LAYOUT, CHUNK, 4>;
+    default:
+      return nullptr;
+  }
+}
+
+template <class LAYOUT>
+AdjKernel pick_adj_chunk(int chunk, int split) {
+  switch (chunk) {
+    case 1:
+      return pick_adj_split<LAYOUT, 1>(split);
+    case 2:
+      return pick_adj_split<LAYOUT, 2>(split);
+    default:
+      return nullptr;
+  }
+}
+
+AdjKernel pick_adjoint(const zodi_model* m, int chunk, int split) {
+  switch (match_layout(m)) {
+    case kLayoutDirbe:
+      return pick_adj_chunk<LayoutDirbe>(chunk, split);
+    case kLayoutKelsall4:
+      return pick_adj_chunk<LayoutKelsall4>(chunk, split);
+    default:
+      return nullptr;
+  }
+}
+
+// Kind-specific parameter count of the gradient layout (kind_nparams on the host).
+int host_kind_nparams(int kind) {
+  switch (kind) {
+    case ZODI_CLOUD:
+      return kind_nparams<ZODI_CLOUD>();
+    case ZODI_BAND:
+      return kind_nparams<ZODI_BAND>();
+    case ZODI_RING:
+      return kind_nparams<ZODI_RING>();
+    case ZODI_FEATURE:
+      return kind_nparams<ZODI_FEATURE>();
+    case ZODI_INTERSTELLAR:
+      return kind_nparams<ZODI_INTERSTELLAR>();
+    case ZODI_RING_RRM:
+      return kind_nparams<ZODI_RING_RRM>();
+    case ZODI_FEATURE_RRM:
+      return kind_nparams<ZODI_FEATURE_RRM>();
+    default:
+      return -1;
+  }
+}
+
+// The kernels index the LDS gradient rows with the grad_* offsets: every slot a
+// component or the shared parameters write must lie inside [0, n_params).
+zodi_status validate_grad_layout(const zodi_model* m) {
+  const int np = m->n_params;
+  for (int c = 0; c < m->n_comps; ++c) {
+    const int k = host_kind_nparams(m->comp_kind[c]);
+    if (k < 0) return fail(ZODI_ERR_MODEL, "component %d kind %d has no adjoint", c,
+                           m->comp_kind[c]);
+    if (m->grad_comp[c] < 0 || m->grad_comp[c] + 5 + k > np)
+      return fail(ZODI_ERR_MODEL, "grad_comp[%d] = %d out of range for %d parameters", c,
+                  m->grad_comp[c], np);
+    if (m->grad_spec_a[c] < 0 || m->grad_spec_a[c] >= np || m->grad_spec_b[c] < 0 ||
+        m->grad_spec_b[c] >= np)
+      return fail(ZODI_ERR_MODEL, "grad_spec_a/b[%d] out of range for %d parameters", c, np);
+  }
+  const int nsh = m->integrand == ZODI_INTEGRAND_KELSALL ? 6 : 1;
+  for (int q = 0; q < nsh; ++q) {
+    if (m->grad_shared[q] < 0 || m->grad_shared[q] >= np)
+      return fail(ZODI_ERR_MODEL, "grad_shared[%d] = %d out of range for %d parameters", q,
+                  m->grad_shared[q], np);
+  }
+  return ZODI_OK;
+}
+
+constexpr int kDefaultChunk = 1;
+constexpr int kDefaultBlocksPerCu = 2;
+
+struct AdjPlan {
+  int chunk;    // components per pass over a group's nodes
+  int split;    // lanes per ray
+  int per_cu;   // resident blocks per CU (grid-stride loop)
+  int64_t n_tiles;
+  int64_t grid;
+};
+
+// Launch plan. With lanes_per_ray = 0 one lane owns a ray unless the rays do not
+// fill every resident wave once; then the smallest split (2 or 4 lanes per ray)
+// that does. Each extra lane per ray repeats the per-ray set-up and the wave
+// reductions, which costs more than a partly idle last round.
+zodi_status adjoint_plan(const zodi_launch* launch, int64_t n_rays, AdjPlan& p) {
+  p.chunk = kDefaultChunk;
+  p.split = 0;
+  p.per_cu = kDefaultBlocksPerCu;
+  if (launch != nullptr) {
+    if (launch->block != 0 && launch->block != kAdjBlock)
+      return fail(ZODI_ERR_ARG, "adjoint block must be %d, got %d", kAdjBlock, launch->block);
+    if (launch->lanes_per_ray != 0 && launch->lanes_per_ray != 1 && launch->lanes_per_ray != 2 &&
+        launch->lanes_per_ray != 4)
+      return fail(ZODI_ERR_ARG, "adjoint lanes_per_ray must be 0, 1, 2 or 4, got %d",
+                  launch->lanes_per_ray);
+    if (launch->blocks_per_cu < 0 || launch->blocks_per_cu > 64)
+      return fail(ZODI_ERR_ARG, "blocks_per_cu %d outside [0, 64]", launch->blocks_per_cu);
+    if (launch->variant != 0 && launch->variant != 1 && launch->variant != 2)
+      return fail(ZODI_ERR_ARG, "adjoint variant (components per pass) must be 0, 1 or 2, got %d",
+                  launch->variant);
+    if (launch->variant != 0) p.chunk = launch->variant;
+    p.split = launch->lanes_per_ray;
+    if (launch->blocks_per_cu > 0) p.per_cu = launch->blocks_per_cu;
+  }
+  const int64_t waves = (int64_t)p.per_cu * cu_count() * kAdjWaves;
+  if (p.split == 0) {
+    p.split = 4;
+    for (int s : {1, 2}) {
+      if ((n_rays * s + kWave - 1) / kWave >= waves) {
+        p.split = s;
+        break;
+      }
+    }
+  }
+  p.n_tiles = (n_rays * p.split + kWave - 1) / kWave;
+  const int64_t blocks = (p.n_tiles + kAdjWaves - 1) / kAdjWaves;
+  p.grid = std::max<int64_t>(1, std::min<int64_t>(blocks, waves / kAdjWaves));
+  return ZODI_OK;
+}
+
+}  // namespace
+}  // namespace zodi
+
+using namespace zodi;
+
+extern "C" {
+
+int64_t zodi_adjoint_workspace(const zodi_model* model_host, int64_t n_rays,
+                               const zodi_launch* launch) {
+  if (model_host == nullptr || validate_model(model_host) != ZODI_OK || n_rays < 0) return -1;
+  AdjPlan p;
+  if (adjoint_plan(launch, n_rays, p) != ZODI_OK) return -1;
+  return (p.grid * (int64_t)(model_host->n_params + 1) + n_rays) * (int64_t)sizeof(double);
+}
+
+zodi_status zodi_adjoint(const zodi_model* model_host, const zodi_model* model_dev,
+                         const double* u, int64_t n_rays, const int64_t* seg_offsets,
+                         int32_t n_seg, const double* obs, const double* earth, int32_t obs_mode,
+                         const uint32_t* beyond, int32_t mode, const double* weights,
+                         const double* target, const double* inv_var, double inv_var_scalar,
+                         double* model_out, double* grad, double* workspace,
+                         int64_t workspace_bytes, const zodi_launch* launch, void* stream) {
+  if (model_host == nullptr) return fail(ZODI_ERR_ARG, "null model");
+  zodi_status st = validate_model(model_host);
+  if (st != ZODI_OK) return st;
+  if (n_rays < 0) return fail(ZODI_ERR_ARG, "n_rays < 0");
+  if (model_dev == nullptr || grad == nullptr || workspace == nullptr)
+    return fail(ZODI_ERR_ARG, "null pointer (model_dev, grad or workspace)");
+  if (n_rays > 0 && (u == nullptr || obs == nullptr || earth == nullptr || beyond == nullptr))
+    return fail(ZODI_ERR_ARG, "null pointer (u, obs, earth or beyond)");
+  if (obs_mode == ZODI_OBS_PER_SEGMENT) {
+    if (n_rays > 0 && (seg_offsets == nullptr || n_seg < 1))
+      return fail(ZODI_ERR_ARG, "per-segment mode needs seg_offsets and n_seg >= 1");
+  } else if (obs_mode != ZODI_OBS_PER_RAY) {
+    return fail(ZODI_ERR_ARG, "unknown obs_mode %d", obs_mode);
+  }
+  if (mode == ZODI_ADJ_CHI2) {
+    if (n_rays > 0 && target == nullptr) return fail(ZODI_ERR_ARG, "chi2 mode needs target");
+  } else if (mode == ZODI_ADJ_VJP_SUM || mode == ZODI_ADJ_VJP_COMPS) {
+    if (n_rays > 0 && weights == nullptr) return fail(ZODI_ERR_ARG, "vjp mode needs weights");
+  } else {
+    return fail(ZODI_ERR_ARG, "unknown adjoint mode %d", mode);
+  }
+  if (model_host->n_params < 1 || model_host->n_params > ZODI_MAX_PARAMS)
+    return fail(ZODI_ERR_MODEL, "n_params %d outside [1, %d]", model_host->n_params,
+                ZODI_MAX_PARAMS);
+  AdjPlan plan;
+  st = adjoint_plan(launch, n_rays, plan);
+  if (st != ZODI_OK) return st;
+  AdjKernel k = pick_adjoint(model_host, plan.chunk, plan.split);
+  if (k == nullptr)
+    return fail(ZODI_ERR_MODEL,
+                "no adjoint for this component layout (supported: cloud, 3 bands[, ring, "
+                "feature] with the Kelsall integrand)");
+  st = validate_grad_layout(model_host);
+  if (st != ZODI_OK) return st;
+  const int np1 = model_host->n_params + 1;
+  const int64_t need = (plan.grid * np1 + n_rays) * (int64_t)sizeof(double);
+  if (workspace_bytes < need)
+    return fail(ZODI_ERR_ARG, "workspace of %lld bytes needed, %lld given", (long long)need,
+                (long long)workspace_bytes);
+  double* vals = model_out != nullptr ? model_out : workspace + plan.grid * np1;
+  if (mode == ZODI_ADJ_CHI2 && n_rays > 0) {
+    st = zodi_forward(model_host, model_dev, u, n_rays, seg_offsets, n_seg, obs, earth, obs_mode,
+                      beyond, vals, ZODI_OUT_SUM, nullptr, stream);
+    if (st != ZODI_OK) return st;
+  }
+  AdjArgs a;
+  memset(&a, 0, sizeof(a));
+  a.f.m = model_dev;
+  a.f.u = u;
+  a.f.n_rays = n_rays;
+  a.f.seg = seg_offsets;
+  a.f.n_seg = n_seg;
+  a.f.obs_mode = obs_mode;
+  a.f.obs = obs;
+  a.f.earth = earth;
+  a.f.obs_stride = obs_mode == ZODI_OBS_PER_RAY ? n_rays : n_seg;
+  a.f.beyond = beyond;
+  a.f.out_mode = ZODI_OUT_SUM;
+  a.f.n_tiles = plan.n_tiles;
+  a.weights = weights;
+  a.target = target;
+  a.inv_var = inv_var;
+  a.inv_var_scalar = inv_var_scalar;
+  a.model_vals = vals;
+  a.partials = workspace;
+  a.n_params = model_host->n_params;
+  a.mode = mode;
+  const size_t lds = (size_t)np1 * kAdjWaves * sizeof(double);
+  hipLaunchKernelGGL(k, dim3((unsigned)plan.grid), dim3(kAdjBlock), lds, (hipStream_t)stream, a);
+  st = check_hip(hipGetLastError(), "adjoint launch");
+  if (st != ZODI_OK) return st;
+  hipLaunchKernelGGL(adjoint_reduce, dim3((np1 + 255) / 256), dim3(256), 0, (hipStream_t)stream,
+                     (const double*)workspace, (int)plan.grid, np1, grad);
+  return check_hip(hipGetLastError(), "adjoint reduce launch");
+}
+
+}  // extern "C"

@@ -1,0 +1,349 @@
+// Shared device and host pieces of the zodiacal kernels (forward and adjoint).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <initializer_list>
+#include <utility>
+
+#include "zodi_device.hpp"
+#include "zodi_hip.h"
+
+namespace zodi {
+
+// Host helpers (defined in zodi_forward.hip).
+zodi_status fail(zodi_status code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
+zodi_status check_hip(hipError_t e, const char* what);
+int cu_count();
+zodi_status validate_model(const zodi_model* m);
+
+// Kernel arguments (passed by value in the kernarg segment).
+struct FwdArgs {
+  const zodi_model* m;
+  const double* u;
+  int64_t n_rays;
+  const int64_t* seg;
+  int32_t n_seg;
+  int32_t obs_mode;
+  const double* obs;
+  const double* earth;
+  int64_t obs_stride;  // row length of obs/earth: n_seg or n_rays
+  const uint32_t* beyond;
+  double* out;
+  int32_t out_mode;
+  int32_t pad;
+  int64_t n_tiles;
+};
+
+// Index of the observer of ray r (largest s with seg[s] <= r); `s0` is that
+// index for an earlier ray of the same tile.
+__device__ __forceinline__ int64_t walk_segment(const int64_t* __restrict__ seg, int32_t n_seg,
+                                                int64_t s0, int64_t r) {
+  int64_t s = s0;
+  while (s + 1 < n_seg && seg[s + 1] <= r) ++s;
+  return s;
+}
+
+__device__ __forceinline__ int64_t find_segment(const int64_t* __restrict__ seg, int32_t n_seg,
+                                                int64_t r) {
+  int32_t lo = 0, hi = n_seg - 1;
+  while (lo < hi) {
+    const int32_t mid = (lo + hi + 1) >> 1;
+    if (seg[mid] <= r) {
+      lo = mid;
+    } else {
+      hi = mid - 1;
+    }
+  }
+  return lo;
+}
+
+// Planck table to LDS as (T_j, slope_j, B_j, 0) entries.
+__device__ __forceinline__ void load_table(ModelPtr M, double* sTab) {
+  const int n = M->n_temps;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    sTab[4 * i] = M->tab_T[i];
+    sTab[4 * i + 1] = M->tab_slope[i];
+    sTab[4 * i + 2] = M->tab_B[i];
+    sTab[4 * i + 3] = 0.0;
+  }
+  __syncthreads();
+}
+
+// Per-ray state shared by every group.
+struct Ray {
+  double ux, uy, uz, ox, oy, oz, ex, ey;
+  RayGeom G;
+  uint32_t bits;
+  int64_t r;
+  bool active;
+};
+
+template <int RAYS>
+__device__ __forceinline__ Ray load_ray(const FwdArgs& a, int64_t tile, int local) {
+  Ray y;
+  const int64_t n = a.n_rays;
+  y.r = tile * RAYS + local;
+  y.active = y.r < n;
+  const int64_t rr = y.active ? y.r : n - 1;
+  int64_t s;
+  if (a.obs_mode == ZODI_OBS_PER_RAY) {
+    s = rr;
+  } else {
+    const int64_t s0 = find_segment(a.seg, a.n_seg, tile * RAYS);
+    s = walk_segment(a.seg, a.n_seg, s0, rr);
+  }
+  y.bits = a.beyond[a.obs_mode == ZODI_OBS_PER_RAY ? 0 : s];
+  const double* __restrict__ U = a.u;
+  y.ux = U[rr];
+  y.uy = U[n + rr];
+  y.uz = U[2 * n + rr];
+  const int64_t os = a.obs_stride;
+  y.ox = a.obs[s];
+  y.oy = a.obs[os + s];
+  y.oz = a.obs[2 * os + s];
+  y.ex = a.earth[s];
+  y.ey = a.earth[os + s];
+  y.G = ray_geometry(y.ux, y.uy, y.uz, y.ox, y.oy, y.oz);
+  return y;
+}
+
+// Line-of-sight range of group g for this ray.
+__device__ __forceinline__ void group_range(ModelPtr M, int g, const Ray& y, double& h,
+                                            double& mid) {
+  const double start = ((y.bits >> (2 * g)) & 1u) ? kEps : sphere_distance(y.G, M->group_r_in[g]);
+  const double stop =
+      ((y.bits >> (2 * g + 1)) & 1u) ? kEps : sphere_distance(y.G, M->group_r_out[g]);
+  h = 0.5 * (stop - start);
+  mid = 0.5 * (stop + start);
+}
+
+// Node geometry of one group: heliocentric position, |X|, 1/|X|, Planck value.
+struct Node {
+  double X0, X1, X2, R, inv_R, B, phi;
+};
+
+struct Tables {
+  const double* tab;
+  double T_first;
+  double T_last;
+  double inv_step;
+  int n_t;
+};
+
+__device__ __forceinline__ Node node_geometry(const Ray& y, double h, double mid, double xk,
+                                              double T0, double mdelta, const Tables& tb) {
+  Node nd;
+  const double s_los = h * xk + mid;
+  nd.X0 = s_los * y.ux + y.ox;
+  nd.X1 = s_los * y.uy + y.oy;
+  nd.X2 = s_los * y.uz + y.oz;
+  norm_and_inverse(nd.X0 * nd.X0 + nd.X1 * nd.X1 + nd.X2 * nd.X2, nd.R, nd.inv_R);
+  // T = T0 R^-delta (zodipy/blackbody.py:16-30), B = interp(T) (zodipy/brightness.py:48)
+  const double T = T0 * pow_radius(nd.R, mdelta);
+  nd.B = planck_lookup(T, tb.tab, tb.n_t, tb.T_first, tb.T_last, tb.inv_step);
+  nd.phi = 0.0;
+  return nd;
+}
+
+// Node contribution of one component to its accumulator.
+// Kelsall (zodipy/brightness.py:21-56): (1 - a) eps B n, plus a F/R^2 Phi n with
+// scattering; without scattering the constant (1 - a) eps is applied once to the
+// node sum (finish). RRM (zodipy/brightness.py:59-83): B n, calibration applied
+// in finish.
+template <int INTEGRAND>
+__device__ __forceinline__ double node_term(ParamPtr P, uint32_t flags, const Node& nd,
+                                            double wk, double wB, double dens, double solar) {
+  if constexpr (INTEGRAND == ZODI_INTEGRAND_KELSALL) {
+    if (flags & ZODI_CF_SCATTER) {
+      double em = P[ZODI_P_ONE_MINUS_ALBEDO] * (P[ZODI_P_EMISSIVITY] * nd.B);
+      em += (P[ZODI_P_ALBEDO] * (solar * (nd.inv_R * nd.inv_R))) * nd.phi;
+      return wk * (em * dens);
+    }
+  }
+  return wB * dens;
+}
+
+// Integrated value of one component from its node sum; h = (stop - start)/2.
+template <int INTEGRAND>
+__device__ __forceinline__ double finish(ParamPtr P, uint32_t flags, double acc, double h,
+                                         double calibration) {
+  if constexpr (INTEGRAND == ZODI_INTEGRAND_KELSALL) {
+    if (flags & ZODI_CF_SCATTER) return __dmul_rn(acc, h);
+    return __dmul_rn(__dmul_rn(acc, P[ZODI_P_SCALE]), h);
+  } else {
+    return __dmul_rn(__dmul_rn(acc, calibration), h);
+  }
+}
+
+// ------------------------------------------------------------ layout kernels
+template <int... Ks>
+struct Grp {
+  static constexpr int size = sizeof...(Ks);
+};
+
+template <int INTEGRAND, class... Gs>
+struct Layout {};
+
+// Registry layouts.
+using LayoutDirbe = Layout<ZODI_INTEGRAND_KELSALL, Grp<ZODI_CLOUD, ZODI_BAND, ZODI_BAND, ZODI_BAND>,
+                           Grp<ZODI_RING>, Grp<ZODI_FEATURE>>;
+using LayoutKelsall4 =
+    Layout<ZODI_INTEGRAND_KELSALL, Grp<ZODI_CLOUD, ZODI_BAND, ZODI_BAND, ZODI_BAND>>;
+using LayoutRRM =
+    Layout<ZODI_INTEGRAND_RRM, Grp<ZODI_FAN>, Grp<ZODI_COMET>,
+           Grp<ZODI_NARROW_BAND, ZODI_NARROW_BAND, ZODI_BROAD_BAND>, Grp<ZODI_INTERSTELLAR>,
+           Grp<ZODI_RING_RRM>, Grp<ZODI_FEATURE_RRM>>;
+
+enum LayoutId { kLayoutGeneric = 0, kLayoutDirbe = 1, kLayoutKelsall4 = 2, kLayoutRRM = 3 };
+
+inline bool layout_is(const zodi_model* m, int integrand, std::initializer_list<std::initializer_list<int>> groups) {
+  if (m->integrand != integrand || m->n_groups != (int)groups.size()) return false;
+  int g = 0, c = 0;
+  for (const auto& grp : groups) {
+    if (m->group_count[g] != (int)grp.size()) return false;
+    for (int kind : grp) {
+      if (m->comp_kind[c++] != kind) return false;
+    }
+    ++g;
+  }
+  return c == m->n_comps;
+}
+
+inline LayoutId match_layout(const zodi_model* m) {
+  if (layout_is(m, ZODI_INTEGRAND_KELSALL,
+                {{ZODI_CLOUD, ZODI_BAND, ZODI_BAND, ZODI_BAND}, {ZODI_RING}, {ZODI_FEATURE}}))
+    return kLayoutDirbe;
+  if (layout_is(m, ZODI_INTEGRAND_KELSALL, {{ZODI_CLOUD, ZODI_BAND, ZODI_BAND, ZODI_BAND}}))
+    return kLayoutKelsall4;
+  if (layout_is(m, ZODI_INTEGRAND_RRM,
+                {{ZODI_FAN},
+                 {ZODI_COMET},
+                 {ZODI_NARROW_BAND, ZODI_NARROW_BAND, ZODI_BROAD_BAND},
+                 {ZODI_INTERSTELLAR},
+                 {ZODI_RING_RRM},
+                 {ZODI_FEATURE_RRM}}))
+    return kLayoutRRM;
+  return kLayoutGeneric;
+}
+
+// ---------------------------------------------------------- per-ray forward evaluation
+// Density of a component whose kind is known at compile time.
+template <int KIND>
+__device__ __forceinline__ double density_k(ParamPtr P, uint32_t flags, const Node& nd,
+                                            double earth_lon) {
+  if constexpr (KIND == ZODI_INTERSTELLAR) {
+    return P[6];
+  } else {
+    const CompFrame f = comp_frame(P, flags, nd.X0, nd.X1, nd.X2, nd.R, nd.inv_R);
+    if constexpr (KIND == ZODI_CLOUD) return cloud_density(P, f);
+    if constexpr (KIND == ZODI_BAND) return band_density(P, flags, f);
+    if constexpr (KIND == ZODI_RING) return ring_density(P, f);
+    if constexpr (KIND == ZODI_FEATURE) return feature_density(P, f, earth_lon);
+    if constexpr (KIND == ZODI_FAN) return fan_density(P, f);
+    if constexpr (KIND == ZODI_COMET) return comet_density(P, f);
+    if constexpr (KIND == ZODI_NARROW_BAND) return narrow_band_density(P, f);
+    if constexpr (KIND == ZODI_BROAD_BAND) return broad_band_density(P, f);
+    if constexpr (KIND == ZODI_RING_RRM) return P[10] * ring_density(P, f);
+    if constexpr (KIND == ZODI_FEATURE_RRM) return P[12] * feature_density(P, f, earth_lon);
+  }
+}
+
+struct Ctx {
+  ModelPtr M;
+  Tables tb;
+  int K;
+  int sub;
+  double solar;
+  double calibration;
+};
+
+template <int INTEGRAND, int L, int... Ks, int... J>
+__device__ __forceinline__ void group_pass(Grp<Ks...>, std::integer_sequence<int, J...>,
+                                           const Ctx& cx, const FwdArgs& a, const Ray& y, int g,
+                                           int first, double& total) {
+  constexpr int C = sizeof...(Ks);
+  constexpr int kinds[C] = {Ks...};
+  const ModelPtr M = cx.M;
+  double h, mid;
+  group_range(M, g, y, h, mid);
+  const double T0 = M->group_T0[g];
+  const double mdelta = -M->group_delta[g];
+
+  double elon[C];
+  ((elon[J] = is_feature(kinds[J]) ? atan2(y.ey - M->comp[first + J][ZODI_P_Y0],
+                                           y.ex - M->comp[first + J][ZODI_P_X0])
+                                   : 0.0),
+   ...);
+  bool scatter = false;
+  if constexpr (INTEGRAND == ZODI_INTEGRAND_KELSALL) {
+    ((scatter = scatter || (M->comp_flags[first + J] & ZODI_CF_SCATTER)), ...);
+  }
+
+  double acc[C] = {};
+  for (int k = cx.sub; k < cx.K; k += L) {
+    const ModelPtr Mk = opaque(M);
+    const double xk = Mk->gl_x[k];
+    const double wk = Mk->gl_w[k];
+    Node nd = node_geometry(y, h, mid, xk, T0, mdelta, cx.tb);
+    if constexpr (INTEGRAND == ZODI_INTEGRAND_KELSALL) {
+      if (scatter) {
+        nd.phi = phase_function(h * xk + mid, y.ux, y.uy, y.uz, nd.X0, nd.X1, nd.X2, nd.R,
+                                Mk->shared);
+      }
+    }
+    const double wB = wk * nd.B;
+    ((acc[J] += node_term<INTEGRAND>(
+          Mk->comp[first + J], Mk->comp_flags[first + J], nd, wk, wB,
+          density_k<kinds[J]>(Mk->comp[first + J], Mk->comp_flags[first + J], nd, elon[J]),
+          Mk->shared[ZODI_S_SOLAR])),
+     ...);
+  }
+  if constexpr (L > 1) {
+#pragma unroll
+    for (int j = 0; j < C; ++j) {
+#pragma unroll
+      for (int off = L / 2; off > 0; off >>= 1) acc[j] += __shfl_xor(acc[j], off, L);
+    }
+  }
+  const bool write = y.active && cx.sub == 0;
+#pragma unroll
+  for (int j = 0; j < C; ++j) {
+    const double e = finish<INTEGRAND>(M->comp[first + j], M->comp_flags[first + j], acc[j], h,
+                                       cx.calibration);
+    if (a.out_mode == ZODI_OUT_COMPS) {
+      if (write) a.out[(int64_t)(first + j) * a.n_rays + y.r] = e;
+    } else {
+      total = __dadd_rn(total, e);
+    }
+  }
+}
+
+template <int L, int INTEGRAND, class... Gs>
+__device__ __forceinline__ void run_layout(Layout<INTEGRAND, Gs...>, const Ctx& cx,
+                                           const FwdArgs& a, const Ray& y, double& total) {
+  int g = 0;
+  int first = 0;
+  ((group_pass<INTEGRAND, L>(Gs{}, std::make_integer_sequence<int, Gs::size>{}, cx, a, y, g,
+                             first, total),
+    ++g, first += Gs::size),
+   ...);
+}
+
+__device__ __forceinline__ Ctx make_ctx(ModelPtr M, const double* sTab, int sub) {
+  Ctx cx;
+  cx.M = M;
+  cx.tb.tab = sTab;
+  cx.tb.T_first = M->tab_T[0];
+  cx.tb.T_last = M->tab_T[M->n_temps - 1];
+  cx.tb.inv_step = M->tab_inv_step;
+  cx.tb.n_t = M->n_temps;
+  cx.K = M->n_nodes;
+  cx.sub = sub;
+  cx.solar = M->shared[ZODI_S_SOLAR];
+  cx.calibration = M->shared[ZODI_S_CALIBRATION];
+  return cx;
+}
+
+
+}  // namespace zodi

@@ -275,7 +275,7 @@ __device__ __forceinline__ double broad_band_density(ParamPtr P, const CompFrame
   return A * pow_radius(f.r * P[14], -gamma) * fv;
 }
 
-__device__ __forceinline__ bool is_feature(int kind) {
+__host__ __device__ constexpr bool is_feature(int kind) {
   return kind == ZODI_FEATURE || kind == ZODI_FEATURE_RRM;
 }
 

@@ -1,0 +1,280 @@
+// Analytic partial derivatives of the component densities (gfx950, FP64).
+//
+// Each function returns the density n at a point together with
+//   dR  = dn/dR'   (R' = |X - X_0|, other arguments held fixed),
+//   dZ  = dn/dZ    (Z = height above the component's symmetry plane),
+//   dth = dn/dtheta_comp (the Earth-trailing feature's longitude, else 0),
+// and the partials with respect to the component's own parameters, in the
+// declaration order of its parameter dataclass after (x_0, y_0, z_0, i, Omega).
+// Angles given in degrees in the parameter dictionary (delta_zeta, theta,
+// sigma_theta) are differentiated per degree. chain_geometry() turns dR, dZ,
+// dth into the partials with respect to x_0, y_0, z_0, i and Omega.
+//
+// The densities are those of zodipy/number_density.py:47-181 and :345-404;
+// |Z| is differentiated as sign(Z) (0 at Z = 0).
+#pragma once
+
+#include "zodi_device.hpp"
+
+namespace zodi {
+
+constexpr double kDegToRad = 0.017453292519943295;  // np.pi / 180 (np.radians)
+
+// Number of kind-specific parameters (after the five geometric ones).
+template <int KIND>
+constexpr int kind_nparams() {
+  if constexpr (KIND == ZODI_CLOUD || KIND == ZODI_BAND) return 5;
+  if constexpr (KIND == ZODI_RING) return 4;
+  if constexpr (KIND == ZODI_FEATURE) return 6;
+  if constexpr (KIND == ZODI_INTERSTELLAR) return 1;
+  if constexpr (KIND == ZODI_RING_RRM) return 5;
+  if constexpr (KIND == ZODI_FEATURE_RRM) return 7;
+  return -1;  // not differentiable here
+}
+
+template <int NP>
+struct DensGrad {
+  double n, dR, dZ, dth;
+  double dp[NP];
+};
+
+__device__ __forceinline__ double sign_of(double z) { return z > 0.0 ? 1.0 : (z < 0.0 ? -1.0 : 0.0); }
+
+// 1/x from the hardware estimate and two Newton steps (each squares the
+// relative error): within an ulp or two for normal x, NaN for x = 0. The
+// derivative terms need no correctly rounded quotient.
+__device__ __forceinline__ double rcp_fast(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  double e = fma(-x, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-x, r, 1.0);
+  return fma(r, e, r);
+}
+
+// Cloud: n_0, alpha, beta, gamma, mu. `logR` is log of the heliocentric
+// distance, which equals log r for a Sun-centred cloud.
+__device__ __forceinline__ DensGrad<5> cloud_grad(ParamPtr P, uint32_t flags, const CompFrame& f,
+                                                  double logR) {
+  const double n0 = P[6], alpha = P[7], beta = P[8], gamma = P[9], mu = P[10];
+  const double inv_2mu = P[13], half_mu = P[14];
+  const double za = fabs(f.zc);
+  const double zeta = za * f.inv_r;
+  const bool inner = zeta < mu;
+  const double g = inner ? (zeta * zeta) * inv_2mu : zeta - half_mu;
+  const double logr = (flags & ZODI_CF_CENTERED) ? logR : log_pos(f.r);
+  const double logg = log_fast(g);
+  const double G = gamma == 0.0 ? 1.0 : exp_fast(gamma * logg);  // g^gamma
+  DensGrad<5> o;
+  const double E = exp_fast(-alpha * logr - beta * G);  // n / n_0
+  o.n = n0 * E;
+  const double dg_dzeta = inner ? zeta * (2.0 * inv_2mu) : 1.0;
+  // gamma g^(gamma-1) written as gamma G / g; its limit at g = 0 is 0 for gamma > 1/2.
+  const double gG = g > 0.0 ? gamma * G * rcp_fast(g) : 0.0;
+  const double dn_dzeta = -o.n * beta * gG * dg_dzeta;
+  o.dR = -alpha * o.n * f.inv_r - dn_dzeta * zeta * f.inv_r;
+  o.dZ = dn_dzeta * sign_of(f.zc) * f.inv_r;
+  o.dth = 0.0;
+  o.dp[0] = E;
+  o.dp[1] = -o.n * logr;
+  o.dp[2] = -o.n * G;
+  o.dp[3] = g > 0.0 ? -o.n * beta * G * logg : 0.0;
+  const double dg_dmu = inner ? -(zeta * zeta) * (2.0 * inv_2mu * inv_2mu) : -0.5;
+  o.dp[4] = -o.n * beta * gG * dg_dmu;
+  return o;
+}
+
+// Band: n_0, delta_zeta [deg], v, p, delta_r.
+__device__ __forceinline__ DensGrad<5> band_grad(ParamPtr P, uint32_t flags, const CompFrame& f) {
+  const double three_n0 = P[6], dzeta = P[7], p = P[8], v = P[9], dr = P[10];
+  const double inv_dzeta = P[13], inv_v = P[14], inv_dr = P[15];
+  const double zd = (fabs(f.zc) * f.inv_r) * inv_dzeta;
+  const double zd2 = zd * zd;
+  const double zd4 = zd2 * zd2;
+  const double zd5 = zd4 * zd;
+  const double zd6 = zd4 * zd2;
+  const double logzd = log_fast(zd);
+  const double zdp = (flags & ZODI_CF_P_IS_4) ? zd4 : (p == 0.0 ? 1.0 : exp_fast(p * logzd));
+  const double t1 = three_n0 * f.inv_r;
+  const double t2 = exp_fast(-zd6);
+  const double t3 = 1.0 + zdp * inv_v;
+  const double x = pow20(f.r * inv_dr);
+  const double ex = exp_fast(-x);
+  const double t4 = 1.0 - ex;
+  DensGrad<5> o;
+  const double t124 = t1 * t2 * t4;
+  o.n = t124 * t3;
+  // d(zd^p)/dzd = p zd^(p-1) (taken as 0 at zd = 0 for p > 1)
+  const double dzdp = (flags & ZODI_CF_P_IS_4) ? 4.0 * (zd2 * zd)
+                                               : (zd > 0.0 ? p * zdp * rcp_fast(zd) : 0.0);
+  const double dn_dzd = t1 * t4 * t2 * (-6.0 * zd5 * t3 + dzdp * inv_v);
+  const double t123_ex = t1 * t2 * t3 * ex;
+  o.dR = -o.n * f.inv_r - dn_dzd * zd * f.inv_r + t123_ex * 20.0 * x * f.inv_r;
+  o.dZ = dn_dzd * sign_of(f.zc) * f.inv_r * inv_dzeta;
+  o.dth = 0.0;
+  o.dp[0] = 3.0 * t2 * t3 * t4 * f.inv_r;
+  o.dp[1] = -dn_dzd * zd * inv_dzeta * kDegToRad;
+  o.dp[2] = -t124 * zdp * inv_v * inv_v;
+  o.dp[3] = zd > 0.0 ? t124 * zdp * logzd * inv_v : 0.0;
+  o.dp[4] = -t123_ex * 20.0 * x * inv_dr;
+  (void)dzeta;
+  (void)v;
+  (void)dr;
+  return o;
+}
+
+// Ring: n_0, R, sigma_r, sigma_z.
+__device__ __forceinline__ DensGrad<4> ring_grad(ParamPtr P, const CompFrame& f) {
+  const double n0 = P[6], Rr = P[7];
+  const double inv_sr2 = P[13], inv_sz = P[14], inv_sr = P[15];
+  const double d = f.r - Rr;
+  const double za = fabs(f.zc);
+  DensGrad<4> o;
+  const double E = exp_fast(-(d * d) * inv_sr2 - za * inv_sz);
+  o.n = n0 * E;
+  o.dR = -2.0 * o.n * d * inv_sr2;
+  o.dZ = -o.n * sign_of(f.zc) * inv_sz;
+  o.dth = 0.0;
+  o.dp[0] = E;
+  o.dp[1] = -o.dR;
+  o.dp[2] = 2.0 * o.n * (d * d) * inv_sr2 * inv_sr;
+  o.dp[3] = o.n * za * (inv_sz * inv_sz);
+  return o;
+}
+
+// Feature: n_0, R, sigma_r, sigma_z, theta [deg], sigma_theta [deg].
+__device__ __forceinline__ DensGrad<6> feature_grad(ParamPtr P, const CompFrame& f,
+                                                    double earth_lon) {
+  const double n0 = P[6], Rf = P[7], th = P[10];
+  const double inv_sr2 = P[13], inv_sz = P[14], inv_sth2 = P[15], inv_sr = P[16], inv_sth = P[17];
+  const double theta_comp = atan2(f.y, f.x) - earth_lon;
+  double dth = theta_comp - th;
+  dth = floor_mod_pos(dth + kPi, kTwoPi) - kPi;
+  const double d = f.r - Rf;
+  const double za = fabs(f.zc);
+  double e = (d * d) * inv_sr2;
+  e += za * inv_sz;
+  e += (dth * dth) * inv_sth2;
+  DensGrad<6> o;
+  const double E = exp_fast(-e);
+  o.n = n0 * E;
+  o.dR = -2.0 * o.n * d * inv_sr2;
+  o.dZ = -o.n * sign_of(f.zc) * inv_sz;
+  o.dth = -2.0 * o.n * dth * inv_sth2;
+  o.dp[0] = E;
+  o.dp[1] = -o.dR;
+  o.dp[2] = 2.0 * o.n * (d * d) * inv_sr2 * inv_sr;
+  o.dp[3] = o.n * za * (inv_sz * inv_sz);
+  o.dp[4] = -o.dth * kDegToRad;
+  o.dp[5] = 2.0 * o.n * (dth * dth) * inv_sth2 * inv_sth * kDegToRad;
+  return o;
+}
+
+// Value and parameter partials of a component whose kind is known at compile time.
+// `logR` = log |X| (heliocentric), reused where the component frame is the Sun's.
+template <int KIND>
+__device__ __forceinline__ DensGrad<kind_nparams<KIND>()> density_grad(ParamPtr P, uint32_t flags,
+                                                                        const CompFrame& f,
+                                                                        double earth_lon,
+                                                                        double logR) {
+  if constexpr (KIND == ZODI_CLOUD) {
+    return cloud_grad(P, flags, f, logR);
+  } else if constexpr (KIND == ZODI_BAND) {
+    return band_grad(P, flags, f);
+  } else if constexpr (KIND == ZODI_RING) {
+    return ring_grad(P, f);
+  } else if constexpr (KIND == ZODI_FEATURE) {
+    return feature_grad(P, f, earth_lon);
+  } else if constexpr (KIND == ZODI_INTERSTELLAR) {
+    DensGrad<1> o;
+    o.n = P[6];
+    o.dR = o.dZ = o.dth = 0.0;
+    o.dp[0] = 1.0;
+    return o;
+  } else if constexpr (KIND == ZODI_RING_RRM) {
+    const DensGrad<4> r = ring_grad(P, f);
+    const double A = P[10];
+    DensGrad<5> o;
+    o.n = A * r.n;
+    o.dR = A * r.dR;
+    o.dZ = A * r.dZ;
+    o.dth = 0.0;
+    for (int k = 0; k < 4; ++k) o.dp[k] = A * r.dp[k];
+    o.dp[4] = r.n;
+    return o;
+  } else {
+    static_assert(KIND == ZODI_FEATURE_RRM, "no derivatives for this component kind");
+    const DensGrad<6> r = feature_grad(P, f, earth_lon);
+    const double A = P[12];
+    DensGrad<7> o;
+    o.n = A * r.n;
+    o.dR = A * r.dR;
+    o.dZ = A * r.dZ;
+    o.dth = A * r.dth;
+    for (int k = 0; k < 6; ++k) o.dp[k] = A * r.dp[k];
+    o.dp[6] = r.n;
+    return o;
+  }
+}
+
+// Earth-direction terms of the feature's longitude for one ray: E' = X_earth - X_0,
+// (E'_y, E'_x) / |E'_xy|^2 (the partials of atan2(E'_y, E'_x) up to sign).
+struct EarthTerm {
+  double gx, gy;
+};
+
+__device__ __forceinline__ EarthTerm earth_term(ParamPtr P, double ex, double ey) {
+  const double epx = ex - P[ZODI_P_X0], epy = ey - P[ZODI_P_Y0];
+  const double inv = 1.0 / (epx * epx + epy * epy);
+  return {epy * inv, epx * inv};
+}
+
+// Partials with respect to x_0, y_0, z_0, i [deg], Omega [deg] from dR, dZ, dth.
+// X' = X - X_0; Z = x' sin(Omega) sin(i) - y' cos(Omega) sin(i) + z' cos(i); for the
+// feature theta_comp = atan2(y', x') - atan2(E'_y, E'_x) with E' = X_earth - X_0.
+template <bool FEATURE>
+__device__ __forceinline__ void chain_geometry(ParamPtr P, const CompFrame& f, double dR,
+                                               double dZ, double dth, const EarthTerm& et,
+                                               double out[5]) {
+  const double si = P[ZODI_P_SIN_I], ci = P[ZODI_P_COS_I];
+  const double so = P[ZODI_P_SIN_OMEGA], co = P[ZODI_P_COS_OMEGA];
+  const double dRr = dR * f.inv_r;
+  out[0] = -(dRr * f.x + dZ * P[ZODI_P_NX]);
+  out[1] = -(dRr * f.y - dZ * P[ZODI_P_NY]);
+  out[2] = -(dRr * f.z + dZ * P[ZODI_P_NZ]);
+  if constexpr (FEATURE) {
+    const double dth_rho2 = dth * rcp_fast(f.x * f.x + f.y * f.y);
+    out[0] += dth_rho2 * f.y - dth * et.gx;
+    out[1] += dth * et.gy - dth_rho2 * f.x;
+  }
+  out[3] = dZ * (f.x * so * ci - f.y * co * ci - f.z * si) * kDegToRad;
+  out[4] = dZ * (f.x * co * si + f.y * so * si) * kDegToRad;
+}
+
+// Scattering phase function and its partials with respect to C1, C2, C3
+// (zodipy/scattering.py:34-59): Phi = N (C1 + C2 Theta + exp(C3 Theta)),
+// N = 1/(2 pi D), D = 2 C1 + pi C2 + (exp(pi C3) + 1)/(C3^2 + 1).
+struct PhaseGrad {
+  double phi, d1, d2, d3;
+};
+
+// cos(Theta) = (X_los . X) / (R_los R) is formed as (u . X) / R with the unit
+// vector u = X_los / R_los, which agrees to rounding.
+__device__ __forceinline__ PhaseGrad phase_grad(double ux, double uy, double uz, double X0,
+                                                double X1, double X2, double inv_R, ParamPtr S) {
+  double ct = (ux * X0 + uy * X1 + uz * X2) * inv_R;
+  ct = ct != ct ? ct : (ct < -1.0 ? -1.0 : (ct > 1.0 ? 1.0 : ct));
+  const double theta = acos(-ct);
+  const double C1 = S[ZODI_S_C1], C2 = S[ZODI_S_C2], C3 = S[ZODI_S_C3];
+  const double N = S[ZODI_S_PHASE_NORM];
+  const double e3 = exp_fast(C3 * theta);
+  const double Pv = C1 + C2 * theta + e3;
+  const double inv_D = kTwoPi * N;  // 1/D
+  PhaseGrad o;
+  o.phi = N * Pv;
+  o.d1 = N - o.phi * 2.0 * inv_D;
+  o.d2 = N * theta - o.phi * kPi * inv_D;
+  o.d3 = N * theta * e3 - o.phi * S[ZODI_S_DD_DC3] * inv_D;
+  return o;
+}
+
+}  // namespace zodi

@@ -1,0 +1,285 @@
+"""Gradients of the line-of-sight integral for zodiacal-model fitting.
+
+The reference evaluates the model only; it has no derivatives. Its parameter
+dictionary (zodipy/model.py:302-333, zodipy/zodiacal_light_model.py:38-55)
+fixes the order of the component parameters here. The parameter vector theta is
+
+* every component's init fields in declaration order (x_0, y_0, z_0, i, Omega,
+  then the kind's own fields; angles in degrees as in the dictionary),
+* Kelsall models: T_0, delta, the effective emissivity and albedo of every
+  component at the model's bandpass, solar irradiance, C1, C2, C3;
+  RRM models: T_0 and delta of every component, calibration.
+
+``ParameterSpace.names`` lists the entries. Gradients come from the
+hand-written adjoint kernel (``zodi_adjoint`` in the HIP library), which
+recomputes the integrand instead of storing node values:
+
+* ``emission(theta, problem)`` -- a ``torch.autograd.Function``: forward is the
+  line-of-sight integral, backward is one adjoint launch with the incoming
+  gradient as weights;
+* ``chi2(theta, problem, data, inv_var)`` -- one library call: the forward
+  kernel writes the model values, then one adjoint launch forms chi^2 and
+  d(chi^2)/d(theta); backward scales the stored gradient.
+
+Supported models: the Kelsall registry layouts (cloud and three bands,
+optionally followed by the ring and the feature). Others raise RuntimeError
+from the library.
+"""
+
+from __future__ import annotations
+
+import dataclasses
+
+import numpy as np
+import torch
+
+from zodipy_amd import _lib
+from zodipy_amd._lib import ADJ_CHI2, ADJ_VJP_COMPS, ADJ_VJP_SUM, OUT_COMPS, OUT_SUM, Launch
+from zodipy_amd.packing import PackedModel, pack_model, param_layout, param_vector
+from zodipy_amd.runtime import DeviceModel, _f64_rows, _stream
+from zodipy_amd.zodiacal_light_model import Integrand
+
+__all__ = ["ParameterSpace", "LineOfSightProblem", "emission", "chi2"]
+
+
+class ParameterSpace:
+    """Map between a parameter vector and the packed model block of one ``Model``.
+
+    The bandpass, quadrature and Planck table of the model stay fixed; only the
+    entries of theta change.
+    """
+
+    def __init__(self, model) -> None:
+        self._ipd = model._ipd_model
+        self._spectral = model._spectral
+        self._b_nu_table = model._b_nu_table
+        self._gl_x, self._gl_w = model._gl_x, model._gl_w
+        self.layout = param_layout(self._ipd)
+        self.names: tuple[str, ...] = tuple(self.layout.names)
+        self.theta0 = param_vector(self._ipd, self._spectral)
+
+    @property
+    def n_params(self) -> int:
+        return len(self.names)
+
+    def index(self, name: str) -> int:
+        return self.names.index(name)
+
+    def _check(self, theta) -> np.ndarray:
+        if isinstance(theta, torch.Tensor):
+            theta = theta.detach().cpu().numpy()
+        theta = np.asarray(theta, dtype=np.float64)
+        if theta.shape != (self.n_params,):
+            msg = f"theta must have shape ({self.n_params},), got {theta.shape}"
+            raise ValueError(msg)
+        return theta
+
+    def pack(self, theta) -> PackedModel:
+        """Packed device block of the model at ``theta``."""
+        theta = self._check(theta)
+        lay = self.layout
+        comps = {}
+        for (label, comp), off in zip(self._ipd.comps.items(), lay.comp):
+            keys = list(comp.init_params())
+            comps[label] = type(comp)(**{k: float(theta[off + i]) for i, k in enumerate(keys)})
+        ipd = dataclasses.replace(self._ipd, comps=comps)
+        spec = {
+            "comp": {lab: dict(v) for lab, v in self._spectral["comp"].items()},
+            "shared": dict(self._spectral["shared"]),
+        }
+        labels = list(self._ipd.comps)
+        sh = spec["shared"]
+        if self._ipd.integrand == Integrand.KELSALL:
+            sh["T_0"], sh["delta"] = float(theta[lay.shared[0]]), float(theta[lay.shared[1]])
+            for c, lab in enumerate(labels):
+                spec["comp"][lab]["emissivity"] = float(theta[lay.spec_a[c]])
+                spec["comp"][lab]["albedo"] = float(theta[lay.spec_b[c]])
+            for k, key in enumerate(("solar_irradiance", "C1", "C2", "C3")):
+                sh[key] = float(theta[lay.shared[2 + k]])
+        else:
+            for c, lab in enumerate(labels):
+                spec["comp"][lab]["T_0"] = float(theta[lay.spec_a[c]])
+                spec["comp"][lab]["delta"] = float(theta[lay.spec_b[c]])
+            sh["calibration"] = float(theta[lay.shared[0]])
+        return pack_model(ipd, spec, self._b_nu_table, self._gl_x, self._gl_w)
+
+
+class LineOfSightProblem:
+    """Rays and observers resident on one GPU, evaluated for any parameter vector.
+
+    ``unit_vectors``: (3, n) ecliptic unit vectors; ``obspos``/``earthpos``:
+    heliocentric ecliptic positions [AU], (3,) for one observer, (3, n_seg) with
+    ``seg_offsets`` (n_seg + 1,) for observers owning consecutive ray ranges, or
+    (3, n) with ``per_ray=True`` (one reference call over time-ordered data).
+    """
+
+    def __init__(self, model, unit_vectors, obspos, earthpos, seg_offsets=None,
+                 per_ray: bool = False) -> None:
+        self.space = ParameterSpace(model)
+        base: DeviceModel = model.device_model
+        self.device = base.device
+        self.lib = _lib.load()
+        self.u = _f64_rows(unit_vectors, 3, "unit vectors", self.device)
+        self.n_rays = int(self.u.shape[1])
+        obs = np.asarray(obspos.cpu() if isinstance(obspos, torch.Tensor) else obspos,
+                         dtype=np.float64).reshape(3, -1)
+        if not per_ray and seg_offsets is None:
+            if obs.shape[1] != 1:
+                msg = "several observers need seg_offsets or per_ray=True"
+                raise ValueError(msg)
+            seg_offsets = np.array([0, self.n_rays], dtype=np.int64)
+        (self.seg, self.n_seg, self.obs, self.earth, self.beyond,
+         self.mode) = base.prepare_observers(obs, earthpos, seg_offsets, per_ray)
+        if self.seg is not None and int(self.seg[-1]) != self.n_rays:
+            msg = f"seg_offsets[-1] = {int(self.seg[-1])} but there are {self.n_rays} rays"
+            raise ValueError(msg)
+        # Adjoint launch (None = library defaults); ``variant`` = components per pass.
+        self.launch: Launch | None = None
+        self._theta_key: bytes | None = None
+        self._block: tuple[np.ndarray, torch.Tensor, PackedModel] | None = None
+        self._workspace: torch.Tensor | None = None
+
+    @property
+    def n_params(self) -> int:
+        return self.space.n_params
+
+    @property
+    def n_comps(self) -> int:
+        return len(self.space._ipd.comps)
+
+    def _model(self, theta):
+        """(host block, device block) of the model at theta (cached for the last theta)."""
+        th = self.space._check(theta)
+        key = th.tobytes()
+        if key != self._theta_key:
+            packed = self.space.pack(th)
+            host = np.ascontiguousarray(packed.block)
+            dev = torch.from_numpy(host.view(np.uint8).copy()).to(self.device)
+            self._block = (host, dev, packed)
+            self._theta_key = key
+        return self._block
+
+    def _ptr_args(self):
+        return (self.u.data_ptr(), self.n_rays,
+                self.seg.data_ptr() if self.seg is not None else None, self.n_seg,
+                self.obs.data_ptr(), self.earth.data_ptr(), self.mode, self.beyond.data_ptr())
+
+    def forward(self, theta, return_comps: bool = False) -> torch.Tensor:
+        """Integrated emission [MJy/sr]: (n_comps, n) or (n,)."""
+        host, dev, _ = self._model(theta)
+        out_mode = OUT_COMPS if return_comps else OUT_SUM
+        shape = (self.n_comps, self.n_rays) if return_comps else (self.n_rays,)
+        out = torch.empty(shape, dtype=torch.float64, device=self.device)
+        st = self.lib.zodi_forward(_lib.host_ptr(host), dev.data_ptr(), *self._ptr_args(),
+                                   out.data_ptr(), out_mode, None, _stream())
+        _lib.check(st, "zodi_forward")
+        return out
+
+    def _ensure_workspace(self, host: np.ndarray) -> torch.Tensor:
+        need = int(self.lib.zodi_adjoint_workspace(_lib.host_ptr(host), self.n_rays,
+                                                   self.launch))
+        if need < 0:
+            err = self.lib.zodi_last_error().decode(errors="replace")
+            msg = f"zodi_adjoint_workspace rejected the model or launch: {err}"
+            raise RuntimeError(msg)
+        n = (need + 7) // 8
+        if self._workspace is None or self._workspace.numel() < n:
+            self._workspace = torch.empty(max(n, 1), dtype=torch.float64, device=self.device)
+        return self._workspace
+
+    def _adjoint(self, theta, mode: int, weights=None, target=None, inv_var=None,
+                 inv_var_scalar: float = 1.0, model_out=None) -> torch.Tensor:
+        host, dev, _ = self._model(theta)
+        ws = self._ensure_workspace(host)
+        grad = torch.empty(self.n_params + 1, dtype=torch.float64, device=self.device)
+
+        def ptr(t):
+            return t.data_ptr() if t is not None else None
+
+        st = self.lib.zodi_adjoint(
+            _lib.host_ptr(host), dev.data_ptr(), *self._ptr_args(), mode, ptr(weights),
+            ptr(target), ptr(inv_var), float(inv_var_scalar), ptr(model_out), grad.data_ptr(),
+            ws.data_ptr(), ws.numel() * 8, self.launch, _stream())
+        _lib.check(st, "zodi_adjoint")
+        return grad
+
+    def _rows(self, t, shape, name) -> torch.Tensor:
+        t = torch.as_tensor(t, dtype=torch.float64, device=self.device)
+        if tuple(t.shape) != shape:
+            msg = f"{name} must have shape {shape}, got {tuple(t.shape)}"
+            raise ValueError(msg)
+        return t.contiguous()
+
+    def vjp(self, theta, weights, return_comps: bool = False) -> torch.Tensor:
+        """sum_r (sum_c) weights * dI/dtheta for weights of the forward output's shape."""
+        if return_comps:
+            w = self._rows(weights, (self.n_comps, self.n_rays), "weights")
+            return self._adjoint(theta, ADJ_VJP_COMPS, weights=w)[:-1]
+        w = self._rows(weights, (self.n_rays,), "weights")
+        return self._adjoint(theta, ADJ_VJP_SUM, weights=w)[:-1]
+
+    def chi2_and_grad(self, theta, data, inv_var=None, return_model: bool = False):
+        """chi^2 = sum_r (I_r - data_r)^2 inv_var_r and its gradient, in one launch.
+
+        ``inv_var``: None (1), a scalar, or (n,) per ray. Returns (chi2 (0-d
+        tensor), grad (n_params,)[, model (n,)]) on the device.
+        """
+        d = self._rows(data, (self.n_rays,), "data")
+        iv_t, iv_s = None, 1.0
+        if inv_var is not None:
+            if np.ndim(inv_var) == 0 and not isinstance(inv_var, torch.Tensor):
+                iv_s = float(inv_var)
+            elif isinstance(inv_var, torch.Tensor) and inv_var.ndim == 0:
+                iv_s = float(inv_var)
+            else:
+                iv_t = self._rows(inv_var, (self.n_rays,), "inv_var")
+        model = (torch.empty(self.n_rays, dtype=torch.float64, device=self.device)
+                 if return_model else None)
+        g = self._adjoint(theta, ADJ_CHI2, target=d, inv_var=iv_t, inv_var_scalar=iv_s,
+                          model_out=model)
+        if return_model:
+            return g[-1], g[:-1], model
+        return g[-1], g[:-1]
+
+
+class _Emission(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, theta, problem, return_comps):
+        ctx.problem = problem
+        ctx.return_comps = return_comps
+        ctx.save_for_backward(theta)
+        return problem.forward(theta.detach(), return_comps)
+
+    @staticmethod
+    def backward(ctx, grad_out):
+        (theta,) = ctx.saved_tensors
+        g = ctx.problem.vjp(theta.detach(), grad_out.to(torch.float64).contiguous(),
+                            ctx.return_comps)
+        return g.to(device=theta.device, dtype=theta.dtype), None, None
+
+
+class _Chi2(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, theta, problem, data, inv_var):
+        value, grad = problem.chi2_and_grad(theta.detach(), data, inv_var)
+        ctx.save_for_backward(grad)
+        ctx.theta_meta = (theta.device, theta.dtype)
+        return value
+
+    @staticmethod
+    def backward(ctx, grad_out):
+        (grad,) = ctx.saved_tensors
+        device, dtype = ctx.theta_meta
+        g = grad * grad_out.to(device=grad.device, dtype=torch.float64)
+        return g.to(device=device, dtype=dtype), None, None, None
+
+
+def emission(theta: torch.Tensor, problem: LineOfSightProblem,
+             return_comps: bool = False) -> torch.Tensor:
+    """Differentiable line-of-sight emission [MJy/sr] at parameter vector theta."""
+    return _Emission.apply(theta, problem, return_comps)
+
+
+def chi2(theta: torch.Tensor, problem: LineOfSightProblem, data, inv_var=None) -> torch.Tensor:
+    """Differentiable chi^2 = sum (I - data)^2 inv_var, forward and gradient fused."""
+    return _Chi2.apply(theta, problem, data, inv_var)

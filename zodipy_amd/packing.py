@@ -50,8 +50,9 @@ MAX_COMPS = 16
 MAX_GROUPS = 16
 MAX_NODES = 512
 N_TEMPS = 100
-COMP_SLOTS = 24
+COMP_SLOTS = 32
 SHARED_SLOTS = 8
+MAX_PARAMS = 256
 MAX_SLOTS_PER_GROUP = 4
 
 # Component kinds (enum in include/zodi_hip.h).
@@ -63,7 +64,8 @@ CF_CENTERED, CF_SCATTER, CF_P_IS_4 = 1, 2, 4
 
 P_X0, P_Y0, P_Z0, P_NX, P_NY, P_NZ, P_K0, P_D0 = 0, 1, 2, 3, 4, 5, 6, 13
 P_EMISSIVITY, P_ALBEDO, P_ONE_MINUS_ALBEDO, P_SCALE = 20, 21, 22, 23
-S_C1, S_C2, S_C3, S_SOLAR, S_PHASE_NORM, S_CALIBRATION = 0, 1, 2, 3, 4, 5
+P_SIN_I, P_COS_I, P_SIN_OMEGA, P_COS_OMEGA = 24, 25, 26, 27
+S_C1, S_C2, S_C3, S_SOLAR, S_PHASE_NORM, S_CALIBRATION, S_DD_DC3 = 0, 1, 2, 3, 4, 5, 6
 
 MODEL_DTYPE = np.dtype(
     [
@@ -79,6 +81,12 @@ MODEL_DTYPE = np.dtype(
         ("comp_flags", "<u4", (MAX_COMPS,)),
         ("group_first", "<i4", (MAX_GROUPS,)),
         ("group_count", "<i4", (MAX_GROUPS,)),
+        ("grad_comp", "<i4", (MAX_COMPS,)),
+        ("grad_spec_a", "<i4", (MAX_COMPS,)),
+        ("grad_spec_b", "<i4", (MAX_COMPS,)),
+        ("grad_shared", "<i4", (8,)),
+        ("n_params", "<i4"),
+        ("reserved2", "<i4"),
         ("group_r_in", "<f8", (MAX_GROUPS,)),
         ("group_r_out", "<f8", (MAX_GROUPS,)),
         ("group_T0", "<f8", (MAX_GROUPS,)),
@@ -118,9 +126,10 @@ def _derived_slots(comp: ZodiacalComponent, kind: int) -> list[float]:
     if kind == KIND_BAND:
         return [1.0 / c.delta_zeta_rad, 1.0 / c.v, 1.0 / c.delta_r]
     if kind in (KIND_RING, KIND_RING_RRM):
-        return [1.0 / c.sigma_r**2, 1.0 / c.sigma_z]
+        return [1.0 / c.sigma_r**2, 1.0 / c.sigma_z, 1.0 / c.sigma_r]
     if kind in (KIND_FEATURE, KIND_FEATURE_RRM):
-        return [1.0 / c.sigma_r**2, 1.0 / c.sigma_z, 1.0 / c.sigma_theta_rad**2]
+        return [1.0 / c.sigma_r**2, 1.0 / c.sigma_z, 1.0 / c.sigma_theta_rad**2,
+                1.0 / c.sigma_r, 1.0 / c.sigma_theta_rad]
     if kind in (KIND_FAN, KIND_COMET):
         return [1.0 / c.Z_0]
     if kind == KIND_NARROW_BAND:
@@ -155,6 +164,72 @@ def _kind_slots(comp: ZodiacalComponent, kind: int) -> list[float]:
     if kind == KIND_FEATURE_RRM:
         return [c.n_0, c.R, c.sigma_r, c.sigma_z, c.theta_rad, c.sigma_theta_rad, c.A]
     raise AssertionError(kind)
+
+
+@dataclass
+class ParamLayout:
+    """Positions of the model parameters in the gradient/parameter vector.
+
+    Order: each component's init fields in declaration order (the order of the
+    reference's parameter dictionary, zodipy/zodiacal_light_model.py:38-55), then
+    Kelsall: T_0, delta, the effective emissivity and albedo of every component,
+    solar irradiance, C1, C2, C3; RRM: T_0 and delta of every component, calibration.
+    """
+
+    names: list[str]
+    comp: list[int]
+    spec_a: list[int]
+    spec_b: list[int]
+    shared: list[int]
+
+    @property
+    def n_params(self) -> int:
+        return len(self.names)
+
+
+def param_layout(model: ZodiacalLightModel) -> ParamLayout:
+    names: list[str] = []
+    comp_off = []
+    for label, comp in model.comps.items():
+        comp_off.append(len(names))
+        names += [f"comps.{label.value}.{k}" for k in comp.init_params()]
+    labels = list(model.comps)
+    if model.integrand == Integrand.KELSALL:
+        shared = [len(names), len(names) + 1]
+        names += ["T_0", "delta"]
+        spec_a = [len(names) + c for c in range(len(labels))]
+        names += [f"emissivity.{lab.value}" for lab in labels]
+        spec_b = [len(names) + c for c in range(len(labels))]
+        names += [f"albedo.{lab.value}" for lab in labels]
+        shared += [len(names) + k for k in range(4)]
+        names += ["solar_irradiance", "C1", "C2", "C3"]
+    else:
+        spec_a = [len(names) + c for c in range(len(labels))]
+        names += [f"T_0.{lab.value}" for lab in labels]
+        spec_b = [len(names) + c for c in range(len(labels))]
+        names += [f"delta.{lab.value}" for lab in labels]
+        shared = [len(names)]
+        names += ["calibration"]
+    return ParamLayout(names, comp_off, spec_a, spec_b, shared)
+
+
+def param_vector(model: ZodiacalLightModel, spectral: dict) -> np.ndarray:
+    """Current values of the parameter vector (see ``param_layout``)."""
+    vals: list[float] = []
+    for comp in model.comps.values():
+        vals += [float(v) for v in comp.init_params().values()]
+    labels = list(model.comps)
+    sh = spectral["shared"]
+    if model.integrand == Integrand.KELSALL:
+        vals += [sh["T_0"], sh["delta"]]
+        vals += [spectral["comp"][lab]["emissivity"] for lab in labels]
+        vals += [spectral["comp"][lab]["albedo"] for lab in labels]
+        vals += [sh["solar_irradiance"], sh["C1"], sh["C2"], sh["C3"]]
+    else:
+        vals += [spectral["comp"][lab]["T_0"] for lab in labels]
+        vals += [spectral["comp"][lab]["delta"] for lab in labels]
+        vals += [sh["calibration"]]
+    return np.asarray(vals, dtype=np.float64)
 
 
 @dataclass
@@ -238,8 +313,11 @@ def pack_model(model: ZodiacalLightModel, spectral: dict, b_nu_table: np.ndarray
         b["shared"][S_C2] = C2
         b["shared"][S_C3] = C3
         b["shared"][S_SOLAR] = shared["solar_irradiance"]
-        any_scatter = any(spectral["comp"][lab]["albedo"] != 0 for lab in labels)
-        b["shared"][S_PHASE_NORM] = phase_normalization(C1, C2, C3) if any_scatter else 0.0
+        # N is used by the forward pass only for components with albedo != 0, and by
+        # the adjoint for d/d(albedo) of every component.
+        b["shared"][S_PHASE_NORM] = phase_normalization(C1, C2, C3)
+        e3 = np.exp(np.pi * C3)
+        b["shared"][S_DD_DC3] = (np.pi * e3 * (C3**2 + 1) - (e3 + 1) * 2 * C3) / (C3**2 + 1) ** 2
     else:
         b["shared"][S_CALIBRATION] = shared["calibration"]
 
@@ -256,6 +334,8 @@ def pack_model(model: ZodiacalLightModel, spectral: dict, b_nu_table: np.ndarray
         slots[P_NX] = comp.sin_Omega_rad * comp.sin_i_rad
         slots[P_NY] = comp.cos_Omega_rad * comp.sin_i_rad
         slots[P_NZ] = comp.cos_i_rad
+        slots[P_SIN_I], slots[P_COS_I] = comp.sin_i_rad, comp.cos_i_rad
+        slots[P_SIN_OMEGA], slots[P_COS_OMEGA] = comp.sin_Omega_rad, comp.cos_Omega_rad
         ks = _kind_slots(comp, kind)
         slots[P_K0:P_K0 + len(ks)] = ks
         ds = _derived_slots(comp, kind)
@@ -301,6 +381,15 @@ def pack_model(model: ZodiacalLightModel, spectral: dict, b_nu_table: np.ndarray
             group_of_comp[c] = g
     if any(f & CF_SCATTER for f in b["comp_flags"][:n_comps]):
         b["flags"] = 1
+    lay = param_layout(model)
+    if lay.n_params > MAX_PARAMS:
+        msg = f"model has {lay.n_params} parameters (max {MAX_PARAMS})"
+        raise ValueError(msg)
+    b["grad_comp"][:n_comps] = lay.comp
+    b["grad_spec_a"][:n_comps] = lay.spec_a
+    b["grad_spec_b"][:n_comps] = lay.spec_b
+    b["grad_shared"][: len(lay.shared)] = lay.shared
+    b["n_params"] = lay.n_params
     return PackedModel(
         block=blk,
         labels=labels,

@@ -102,8 +102,9 @@ class DeviceModel:
             beyond = self.packed.beyond_bits(obs_np, any_over_all=False)
             seg = torch.from_numpy(seg_np).to(self.device)
             mode = OBS_PER_SEGMENT
-        obs = torch.from_numpy(np.ascontiguousarray(obs_np)).to(self.device)
-        earth = torch.from_numpy(np.ascontiguousarray(earth_np)).to(self.device)
+        # np.array copies: broadcast views (per-ray observers) are read-only
+        obs = torch.from_numpy(np.array(obs_np, dtype=np.float64, order="C")).to(self.device)
+        earth = torch.from_numpy(np.array(earth_np, dtype=np.float64, order="C")).to(self.device)
         beyond_t = torch.from_numpy(beyond.view(np.int32).copy()).to(self.device)
         return seg, n_seg, obs, earth, beyond_t, mode
 

@@ -77,8 +77,18 @@ def _sorted_grid(spectrum: u.Quantity, table):
     return grid, vals
 
 
-def interp_linear(grid: np.ndarray, vals: np.ndarray, xq: np.ndarray) -> np.ndarray:
-    """Piecewise-linear interpolation; the end segments extend beyond the grid."""
+def interp_linear(grid: np.ndarray, vals: np.ndarray, xq: np.ndarray,
+                  extrapolate: bool = False) -> np.ndarray:
+    """Piecewise-linear interpolation as SciPy's ``interp1d(kind="linear")`` rounds it.
+
+    Inside the grid (no extrapolation) SciPy hands float64 data to ``np.interp``,
+    which uses the segment whose left node is <= x and returns node values
+    exactly; with ``fill_value="extrapolate"`` it evaluates
+    ``slope * (x - x_lo) + y_lo`` on the segment whose right node is >= x, the end
+    segments extended beyond the grid.
+    """
+    if not extrapolate:
+        return np.interp(xq, grid, vals)
     idx = np.clip(np.searchsorted(grid, xq, side="left"), 1, grid.size - 1)
     lo, hi = idx - 1, idx
     slope = (vals[hi] - vals[lo]) / (grid[hi] - grid[lo])
@@ -105,7 +115,8 @@ def effective_param(x_model: np.ndarray, weights, spectrum: u.Quantity, table,
     if not extrapolate and (np.any(xq < grid[0]) or np.any(xq > grid[-1])):
         msg = "A value in x_new is outside the interpolation range."
         raise ValueError(msg)
-    p = interp_nearest(grid, vals, xq) if nearest else interp_linear(grid, vals, xq)
+    p = (interp_nearest(grid, vals, xq) if nearest
+         else interp_linear(grid, vals, xq, extrapolate))
     if weights is not None:
         return trapezoid(np.asarray(weights) * p, x=xq)
     return p
