@@ -1,0 +1,312 @@
+"""Host-side logic: the C ABI library, the model block packer, the parameter
+space and the reference's front-end behaviour (tests/test_model.py of zodipy).
+
+CPU only. Calls into the library are restricted to entry points and argument
+errors that return before any device work; those tests are skipped where a GPU
+is present, since their device pointers are placeholders.
+"""
+
+from __future__ import annotations
+
+import ctypes as ct
+import re
+import subprocess
+import sys
+from pathlib import Path
+
+import numpy as np
+import pytest
+import torch
+
+from zodipy_amd import Model, _lib, healpix
+from zodipy_amd import units as u
+from zodipy_amd.component import ComponentLabel
+from zodipy_amd.fitting import ParameterSpace
+from zodipy_amd.packing import (
+    CF_CENTERED,
+    CF_P_IS_4,
+    CF_SCATTER,
+    MODEL_DTYPE,
+    P_EMISSIVITY,
+    P_SCALE,
+    pack_model,
+    param_layout,
+)
+from zodipy_amd.zodiacal_light_model import ZodiacalLightModel, model_registry
+
+ROOT = Path(__file__).resolve().parents[1]
+HEADER = ROOT / "include" / "zodi_hip.h"
+NO_GPU = not torch.cuda.is_available()
+cpu_only = pytest.mark.skipif(not NO_GPU, reason="placeholder device pointers: CPU-only check")
+
+KELSALL_MODELS = ["dirbe", "planck13", "planck15", "planck18", "odegard"]
+
+
+def header_functions() -> set[str]:
+    text = HEADER.read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return set(re.findall(r"\b(zodi_\w+)\s*\(", text))
+
+
+# --------------------------------------------------------------------- C ABI
+def test_library_exports_every_declared_function():
+    lib = _lib.load()
+    declared = header_functions()
+    assert declared == set(_lib.EXPORTED_SYMBOLS)
+    for name in declared:
+        assert hasattr(lib, name), name
+
+
+def test_abi_version_and_model_size():
+    lib = _lib.load()
+    assert lib.zodi_abi_version() == 1
+    assert lib.zodi_model_size() == MODEL_DTYPE.itemsize == 15848
+
+
+def _packed(name="dirbe", x=25 * u.micron, **kw):
+    return Model(x, name=name, **kw)._packed
+
+
+def _call_adjoint(lib, block, mode=0, launch=None, ws_bytes=1 << 20, n_rays=4):
+    dummy = ct.c_void_p(0x1000)  # never dereferenced: every case fails host validation
+    grad = ct.c_void_p(0x2000)
+    return lib.zodi_adjoint(block.ctypes.data, dummy, dummy, n_rays, dummy, 1, dummy, dummy, 0,
+                            dummy, mode, dummy, dummy, None, 1.0, None, grad, dummy, ws_bytes,
+                            launch, None)
+
+
+@cpu_only
+def test_host_validation_errors():
+    lib = _lib.load()
+    blk = np.ascontiguousarray(_packed().block)
+    dummy = ct.c_void_p(0x1000)
+    # zodi_forward: null model, wrong ABI version, negative ray count, bad modes
+    assert lib.zodi_forward(None, dummy, dummy, 1, dummy, 1, dummy, dummy, 0, dummy, dummy, 0,
+                            None, None) == 1
+    bad = blk.copy()
+    bad["abi_version"] = 7
+    assert lib.zodi_forward(bad.ctypes.data, dummy, dummy, 1, dummy, 1, dummy, dummy, 0, dummy,
+                            dummy, 0, None, None) == 2
+    assert b"abi_version" in lib.zodi_last_error()
+    assert lib.zodi_forward(blk.ctypes.data, dummy, dummy, -1, dummy, 1, dummy, dummy, 0, dummy,
+                            dummy, 0, None, None) == 1
+    assert lib.zodi_forward(blk.ctypes.data, dummy, dummy, 1, dummy, 1, dummy, dummy, 5, dummy,
+                            dummy, 0, None, None) == 1
+    assert lib.zodi_forward(blk.ctypes.data, dummy, dummy, 1, dummy, 1, dummy, dummy, 0, dummy,
+                            dummy, 9, None, None) == 1
+    groups = blk.copy()
+    groups["group_count"][0, 0] = 3  # groups no longer cover the components
+    assert lib.zodi_forward(groups.ctypes.data, dummy, dummy, 1, dummy, 1, dummy, dummy, 0,
+                            dummy, dummy, 0, None, None) == 2
+
+
+@cpu_only
+def test_adjoint_host_validation_errors():
+    lib = _lib.load()
+    blk = np.ascontiguousarray(_packed().block)
+    assert _call_adjoint(lib, blk, mode=7) == 1
+    assert b"mode" in lib.zodi_last_error()
+    assert _call_adjoint(lib, blk, launch=ct.byref(_lib.Launch(128, 0, 0, 0))) == 1
+    assert _call_adjoint(lib, blk, launch=ct.byref(_lib.Launch(0, 3, 0, 0))) == 1
+    assert _call_adjoint(lib, blk, launch=ct.byref(_lib.Launch(0, 0, 0, 5))) == 1
+    assert _call_adjoint(lib, blk, ws_bytes=8) == 1
+    assert b"workspace" in lib.zodi_last_error()
+    corrupt = blk.copy()
+    corrupt["grad_comp"][0, 1] = 75  # band1's ten slots would pass the end of 78 parameters
+    assert _call_adjoint(lib, corrupt) == 2
+    assert b"grad_comp" in lib.zodi_last_error()
+    corrupt = blk.copy()
+    corrupt["grad_shared"][0, 5] = -1
+    assert _call_adjoint(lib, corrupt) == 2
+    rrm = np.ascontiguousarray(_packed("rrm-experimental", 1.25 * u.micron,
+                                       extrapolate=True).block)
+    assert _call_adjoint(lib, rrm) == 2
+    assert b"no adjoint" in lib.zodi_last_error()
+
+
+@cpu_only
+def test_adjoint_workspace_size():
+    lib = _lib.load()
+    blk = np.ascontiguousarray(_packed().block)
+    # without a device the library assumes 256 CUs: grid = min(tiles / 4, 2 * 256) blocks
+    n = 1_000_000
+    got = lib.zodi_adjoint_workspace(blk.ctypes.data, n, None)
+    assert got == (512 * 79 + n) * 8
+    assert lib.zodi_adjoint_workspace(blk.ctypes.data, 64, None) == (1 * 79 + 64) * 8
+    assert lib.zodi_adjoint_workspace(blk.ctypes.data, -1, None) == -1
+    assert lib.zodi_adjoint_workspace(blk.ctypes.data, n, ct.byref(_lib.Launch(0, 3, 0, 0))) == -1
+
+
+# --------------------------------------------------------------------- packer
+def test_dirbe_block_groups_and_flags():
+    pk = _packed()
+    b = pk.block[0]
+    assert b["n_comps"] == 6 and b["n_groups"] == 3
+    assert list(b["group_count"][:3]) == [4, 1, 1]
+    assert list(b["group_first"][:3]) == [0, 4, 5]
+    flags = b["comp_flags"][:6]
+    assert all(f & CF_CENTERED for f in flags[1:])  # bands, ring, feature sit at the Sun
+    assert not flags[0] & CF_CENTERED  # the cloud is offset
+    assert all(f & CF_P_IS_4 for f in flags[1:4])
+    assert not any(f & CF_SCATTER for f in flags)  # albedo 0 at 25 um
+    assert b["n_nodes"] == 50 and b["n_params"] == 78
+    scat = _packed(x=1.25 * u.micron).block[0]
+    assert all(f & CF_SCATTER for f in scat["comp_flags"][:6])
+
+
+def test_beyond_bits_follow_cutoffs():
+    pk = _packed()
+    obs = np.array([[1.0, 1.25, 0.5], [0.0, 0.0, 0.0], [0.0, 0.0, 0.0]])
+    bits = pk.beyond_bits(obs, any_over_all=False)
+    # groups: 0 = cloud+bands (eps, 5.2), 1 = ring (0.8, 1.2), 2 = feature (0.7, 1.3);
+    # bit 2g: beyond group g's start sphere, bit 2g+1: beyond its stop sphere
+    assert bits[0] == 1 | (1 << 2) | (1 << 4)             # beyond every start sphere
+    assert bits[1] == 1 | (1 << 2) | (1 << 3) | (1 << 4)  # also beyond the ring's stop
+    assert bits[2] == 1                                   # inside all but the eps sphere
+    allb = pk.beyond_bits(obs, any_over_all=True)
+    assert np.all(allb == (bits[0] | bits[1] | bits[2]))
+
+
+def test_param_layout_dirbe():
+    lay = param_layout(model_registry.get_model("dirbe"))
+    assert lay.n_params == 78
+    assert lay.names[:5] == ["comps.cloud.x_0", "comps.cloud.y_0", "comps.cloud.z_0",
+                             "comps.cloud.i", "comps.cloud.Omega"]
+    assert lay.names[lay.comp[1] + 5] == "comps.band1.n_0"
+    assert lay.names[lay.shared[0]] == "T_0" and lay.names[lay.shared[5]] == "C3"
+    assert lay.names[lay.spec_a[4]] == "emissivity.ring"
+    assert lay.names[lay.spec_b[5]] == "albedo.feature"
+    assert len(set(lay.comp + lay.spec_a + lay.spec_b + lay.shared)) == 6 * 3 + 6
+
+
+@pytest.mark.parametrize("name", KELSALL_MODELS)
+def test_parameter_space_round_trip(name):
+    x = 857 * u.GHz if name.startswith("planck") else 25 * u.micron
+    model = Model(x, name=name, extrapolate=True)
+    sp = ParameterSpace(model)
+    assert sp.pack(sp.theta0).block.tobytes() == model._packed.block.tobytes()
+
+
+def test_parameter_space_moves_only_its_slots():
+    model = Model(25 * u.micron)
+    sp = ParameterSpace(model)
+    th = sp.theta0.copy()
+    k = sp.index("emissivity.band2")
+    th[k] *= 1.5
+    a, b = model._packed.block[0], sp.pack(th).block[0]
+    diff = np.argwhere(a["comp"] != b["comp"])
+    assert sorted(map(tuple, diff)) == [(2, P_EMISSIVITY), (2, P_SCALE)]
+    for f in MODEL_DTYPE.names:
+        if f != "comp":
+            assert np.array_equal(a[f], b[f]), f
+
+
+def test_model_block_matches_header_layout():
+    """Field offsets of the packer's dtype follow the struct in include/zodi_hip.h."""
+    text = re.sub(r"/\*.*?\*/", "", HEADER.read_text(), flags=re.S)
+    body = text[text.index("typedef struct zodi_model {"):text.index("} zodi_model;")]
+    fields = re.findall(r"\b(?:int32_t|uint32_t|double)\s+(\w+)", body)
+    assert fields == list(MODEL_DTYPE.names)
+
+
+# --------------------------------------------------------------------- front end
+def test_x_input():
+    with pytest.raises(u.UnitConversionError):
+        Model(x=20 * u.s)
+    with pytest.raises(TypeError):
+        Model(x=20)
+    with pytest.raises(ValueError):
+        Model(np.linspace(20, 25, 10) * u.micron)
+
+
+def test_weights_input():
+    bandpass = np.random.default_rng(0).random(10)
+    with pytest.raises(ValueError):
+        Model(20 * u.micron, weights=bandpass)
+    with pytest.raises(ValueError):
+        Model(np.linspace(20, 25, 15) * u.micron, weights=bandpass, name="dirbe")
+
+
+def test_extrapolate_input():
+    with pytest.raises(ValueError):
+        Model(0.5 * u.micron, name="dirbe", extrapolate=False)
+    with pytest.raises(ValueError):
+        Model(400 * u.micron, name="dirbe", extrapolate=False)
+    Model(20 * u.micron, extrapolate=True)
+    x = np.linspace(40, 125, 10) * u.GHz
+    bandpass = np.random.default_rng(1).random(10)
+    with pytest.raises(ValueError):
+        Model(x, weights=bandpass, name="planck18", extrapolate=False)
+    Model(x, weights=bandpass, name="planck18", extrapolate=True)
+
+
+def test_parameters_round_trip():
+    model = Model(20 * u.micron, name="dirbe")
+    params = model.get_parameters()
+    assert isinstance(params, dict)
+    before = model._packed.block.tobytes()
+    model.update_parameters(params)
+    assert model._packed.block.tobytes() == before
+    params["comps"]["ring"]["sigma_r"] = 0.5
+    model.update_parameters(params)
+    assert model._packed.block.tobytes() != before
+
+
+def test_registry():
+    with pytest.raises(ValueError):
+        model_registry.get_model("invalid")
+    with pytest.raises(ValueError):
+        model_registry.register_model("DIRBE", model_registry.get_model("DIRBE"))
+    with pytest.raises(TypeError):
+        model_registry.register_model("not a model", 2)
+    for name in ["dirbe", "planck13", "planck15", "planck18", "odegard", "rrm-experimental"]:
+        assert isinstance(model_registry.get_model(name), ZodiacalLightModel)
+
+
+def test_pack_rejects_unknown_component_kind():
+    import dataclasses
+
+    dirbe = model_registry.get_model("dirbe")
+
+    class Odd:
+        pass
+
+    odd = dataclasses.replace(dirbe, comps={ComponentLabel.CLOUD: Odd()})
+    model = Model(25 * u.micron)
+    with pytest.raises((TypeError, AttributeError)):
+        pack_model(odd, model._spectral, model._b_nu_table, model._gl_x, model._gl_w)
+
+
+def test_no_gpu_means_loud_failure():
+    if not NO_GPU:
+        pytest.skip("a GPU is present")
+    model = Model(25 * u.micron)
+    with pytest.raises(RuntimeError, match="GPU"):
+        model.evaluate_vectors(np.array([1.0, 0.0, 0.0]), obspos=[1.0, 0, 0], earthpos=[1.0, 0, 0])
+
+
+# --------------------------------------------------------------------- HEALPix
+@pytest.mark.parametrize("nside", [1, 4, 16])
+def test_pix2vec_ring_properties(nside):
+    v = np.asarray(healpix.pix2vec(nside))
+    npix = 12 * nside * nside
+    assert v.shape == (3, npix)
+    np.testing.assert_allclose(np.linalg.norm(v, axis=0), 1.0, rtol=0, atol=1e-15)
+    np.testing.assert_allclose(v.mean(axis=1), 0.0, atol=1e-12)
+    # RING scheme (Gorski et al. 2005, ApJ 622, 759, eqs. 4-5): 4 nside - 1 rings, z
+    # non-increasing with the pixel index, ring i at z = 1 - i^2 / (3 nside^2) in the
+    # polar caps (i < nside) and z = 4/3 - 2 i / (3 nside) in the equatorial belt
+    assert np.all(np.diff(v[2]) <= 1e-15)
+    i = np.arange(1, 4 * nside)
+    z_ring = np.where(i < nside, 1 - i**2 / (3 * nside**2), 4 / 3 - 2 * i / (3 * nside))
+    z_ring = np.where(i > 3 * nside, -(1 - (4 * nside - i) ** 2 / (3 * nside**2)), z_ring)
+    zs = np.unique(np.round(v[2], 12))[::-1]
+    np.testing.assert_allclose(zs, z_ring, rtol=0, atol=1e-12)
+
+
+# --------------------------------------------------------------------- bench contract
+def test_bench_refuses_more_than_one_gpu():
+    r = subprocess.run([sys.executable, str(ROOT / "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode != 0
+    assert "one GPU" in r.stderr
