@@ -234,7 +234,7 @@ enum {
  * zodi_adjoint — gradient of sum_r sum_c weights_c,r I_c,r (or of chi2) with respect
  * to the parameter vector whose layout the model block's grad_* fields give.
  * No reference counterpart: the reference has no derivatives; its parameter
- * dictionary (zodipy/zodiacal_light_model.py:38-55, zodipy/model.py:302-333) fixes
+ * dictionary (zodipy/zodiacal_light_model.py:38-55, zodipy/model.py:303-333) fixes
  * the order of the component entries.
  *
  * Rays are addressed as in zodi_forward. grad is [n_params + 1] in device memory;

@@ -1,7 +1,7 @@
 """Gradients of the line-of-sight integral for zodiacal-model fitting.
 
 The reference evaluates the model only; it has no derivatives. Its parameter
-dictionary (zodipy/model.py:302-333, zodipy/zodiacal_light_model.py:38-55)
+dictionary (zodipy/model.py:303-333, zodipy/zodiacal_light_model.py:38-55)
 fixes the order of the component parameters here. The parameter vector theta is
 
 * every component's init fields in declaration order (x_0, y_0, z_0, i, Omega,
