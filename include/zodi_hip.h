@@ -222,6 +222,16 @@ zodi_status zodi_number_density(const zodi_model* model_host, const zodi_model* 
                                 const double* X, int64_t n_points,
                                 const double* earth, double* out, void* stream);
 
+/*
+ * zodi_host_device_pointer — the device address of page-locked, mapped host
+ * memory (hipHostMalloc, torch's pin_memory), so the kernels can read pointings
+ * from, and write results to, host memory directly (zero-copy batches: one pass
+ * over PCIe inside the kernel instead of separate copies). Fails with
+ * ZODI_ERR_ARG for memory that is not page-locked host memory. No reference
+ * counterpart: a helper for callers whose data stays in host memory.
+ */
+zodi_status zodi_host_device_pointer(const void* host, void** device);
+
 /* Adjoint modes. */
 enum {
   ZODI_ADJ_VJP_SUM = 0,    /* weights [n_rays] against the component sum */
@@ -245,7 +255,7 @@ enum {
  * bytes of device memory for the same n_rays and launch. launch may be NULL;
  * otherwise block must be 0 or 256, lanes_per_ray 0 (chosen from n_rays), 1, 2 or
  * 4 (lanes sharing one ray's nodes), blocks_per_cu 0..64 (0 = 2), and variant the
- * components evaluated per pass over a group's nodes: 0 (default, 1), 1 or 2.
+ * components evaluated per pass over a group's nodes: 0 (default, 2), 1 or 2.
  * Supported layouts: the Kelsall registry models (cloud and three bands,
  * optionally followed by the ring and the feature); others return
  * ZODI_ERR_MODEL. The reduction order is fixed: equal inputs give equal bits.

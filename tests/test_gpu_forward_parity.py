@@ -143,9 +143,11 @@ def test_batched_segments_uneven(gpu):
     assert np.array_equal(tot, got.sum(axis=0))
 
 
+@pytest.mark.parametrize("variant", [0, 1])
 @pytest.mark.parametrize("block,lanes,per_cu", [(64, 1, 0), (128, 2, 0), (256, 4, 2),
                                                 (512, 8, 1), (256, 1, 4)])
-def test_launch_variants_agree(gpu, block, lanes, per_cu):
+def test_launch_variants_agree(gpu, block, lanes, per_cu, variant):
+    """Generic (variant 0) and layout-specialised (variant 1) kernels, every launch shape."""
     rng = np.random.default_rng(2)
     counts = rng.integers(1, 400, 9)
     seg = np.concatenate([[0], np.cumsum(counts)]).astype(np.int64)
@@ -154,7 +156,7 @@ def test_launch_variants_agree(gpu, block, lanes, per_cu):
     obs = np.stack([np.cos(ph), np.sin(ph), np.zeros_like(ph)])
     model = Model(25 * u.micron)
     dm = model.device_model
-    launch = Launch(block=block, lanes_per_ray=lanes, blocks_per_cu=per_cu, variant=0)
+    launch = Launch(block=block, lanes_per_ray=lanes, blocks_per_cu=per_cu, variant=variant)
     got = dm.forward(v, obs, obs, seg_offsets=seg, launch=launch).cpu().numpy()
     want = evaluate_segments(OracleModel(25 * u.micron), v, seg, obs, obs)
     assert_parity(got, want)
@@ -238,3 +240,88 @@ def test_device_tensor_output_and_stream(gpu):
     assert out.is_cuda and out.shape == (4096,)
     want = OracleModel(25 * u.micron).evaluate_vectors(v.cpu().numpy(), obs, obs).sum(axis=0)
     assert_parity(out.cpu().numpy(), want)
+
+
+def _uneven_batch(seed, n_obs=17):
+    rng = np.random.default_rng(seed)
+    counts = rng.integers(0, 900, n_obs)
+    counts[[2, 9]] = 0
+    seg = np.concatenate([[0], np.cumsum(counts)]).astype(np.int64)
+    ph = rng.uniform(0, 2 * np.pi, n_obs)
+    obs = np.stack([np.cos(ph), np.sin(ph), 0.003 * np.cos(3 * ph)])
+    obs[:, 5] *= 1.25  # outside the ring's stop sphere
+    earth = np.stack([np.cos(ph), np.sin(ph), np.zeros_like(ph)])
+    v = random_dirs(int(seg[-1]), seed=seed + 1)
+    return v, seg, obs, earth
+
+
+@pytest.mark.parametrize("n_chunks", [1, 3, 7])
+@pytest.mark.parametrize("return_comps", [False, True])
+def test_streamed_evaluator_matches_resident_batch(gpu, n_chunks, return_comps):
+    """Host-to-host pipelined chunks (cuts inside segments) give the resident batch's bits."""
+    from zodipy_amd.batch import BatchedEvaluator, StreamedEvaluator
+
+    v, seg, obs, earth = _uneven_batch(31)
+    dm = Model(12 * u.micron).device_model
+    u_dev = torch.as_tensor(v, device="cuda").contiguous()
+    want = BatchedEvaluator(dm, u_dev, seg, obs, earth, return_comps=return_comps,
+                            use_graph=False).run().cpu()
+    st = StreamedEvaluator(dm, seg, obs, earth, n_chunks=n_chunks, return_comps=return_comps)
+    u_host = torch.as_tensor(v).pin_memory()
+    out_host = torch.empty(tuple(want.shape), dtype=torch.float64).pin_memory()
+    for _ in range(2):  # the second run reuses the buffers
+        out_host.fill_(-1.0)
+        st.run(u_host, out_host)
+        torch.cuda.synchronize()
+        assert torch.equal(out_host, want)
+    ref = evaluate_segments(OracleModel(12 * u.micron), v, seg, obs, earth)
+    assert_parity(want.numpy(), ref if return_comps else ref.sum(axis=0))
+
+
+def test_batched_graph_replays_match_eager(gpu):
+    """Graph replays, with eager work and synchronisation between them, match eager launches."""
+    from zodipy_amd.batch import BatchedEvaluator
+
+    v, seg, obs, earth = _uneven_batch(41)
+    dm = Model(25 * u.micron).device_model
+    u_dev = torch.as_tensor(v, device="cuda").contiguous()
+    eager = BatchedEvaluator(dm, u_dev, seg, obs, earth, use_graph=False).run().clone()
+    graphed = BatchedEvaluator(dm, u_dev, seg, obs, earth, use_graph=True)
+    for k in range(4):
+        graphed.out.fill_(float(k))
+        out = graphed.run()
+        torch.cuda.synchronize()
+        assert torch.equal(out, eager)
+        scratch = torch.randn(1000, device="cuda").sum()  # unrelated eager work between replays
+        torch.cuda.synchronize()
+        assert torch.isfinite(scratch)
+
+
+@pytest.mark.parametrize("return_comps", [False, True])
+def test_host_mapped_evaluator_matches_resident_batch(gpu, return_comps):
+    """Zero-copy: the kernel reads pinned host pointings and writes pinned host results."""
+    from zodipy_amd.batch import BatchedEvaluator, HostMappedEvaluator
+
+    v, seg, obs, earth = _uneven_batch(51)
+    dm = Model(12 * u.micron).device_model
+    u_dev = torch.as_tensor(v, device="cuda").contiguous()
+    want = BatchedEvaluator(dm, u_dev, seg, obs, earth, return_comps=return_comps,
+                            use_graph=False).run().cpu()
+    hm = HostMappedEvaluator(dm, seg, obs, earth, return_comps=return_comps)
+    u_host = torch.as_tensor(v).contiguous().pin_memory()
+    out_host = torch.empty(tuple(want.shape), dtype=torch.float64).pin_memory()
+    for _ in range(2):
+        out_host.fill_(-1.0)
+        hm.run(u_host, out_host)
+        torch.cuda.synchronize()
+        assert torch.equal(out_host, want)
+
+
+def test_host_mapped_evaluator_rejects_pageable_memory(gpu):
+    from zodipy_amd.batch import HostMappedEvaluator
+
+    v, seg, obs, earth = _uneven_batch(52)
+    hm = HostMappedEvaluator(Model(12 * u.micron).device_model, seg, obs, earth)
+    out = torch.empty(int(seg[-1]), dtype=torch.float64).pin_memory()
+    with pytest.raises(ValueError, match="pinned"):
+        hm.run(torch.as_tensor(v).contiguous(), out)  # pageable pointings

@@ -94,6 +94,10 @@ def test_host_validation_errors():
                             dummy, 0, None, None) == 1
     assert lib.zodi_forward(blk.ctypes.data, dummy, dummy, 1, dummy, 1, dummy, dummy, 0, dummy,
                             dummy, 9, None, None) == 1
+    out = ct.c_void_p()
+    pageable = np.zeros(16)
+    assert lib.zodi_host_device_pointer(pageable.ctypes.data, ct.byref(out)) == 1
+    assert lib.zodi_host_device_pointer(None, ct.byref(out)) == 1
     groups = blk.copy()
     groups["group_count"][0, 0] = 3  # groups no longer cover the components
     assert lib.zodi_forward(groups.ctypes.data, dummy, dummy, 1, dummy, 1, dummy, dummy, 0,

@@ -27,6 +27,7 @@ EXPORTED_SYMBOLS = (
     "zodi_forward",
     "zodi_line_of_sight",
     "zodi_number_density",
+    "zodi_host_device_pointer",
     "zodi_adjoint_workspace",
     "zodi_adjoint",
 )
@@ -64,6 +65,8 @@ def _bind(lib: ct.CDLL) -> None:
     lib.zodi_line_of_sight.argtypes = [_p, _p, _p, _i64, _p, _i32, _p, _i32, _p, _p, _p, _p]
     lib.zodi_number_density.restype = ct.c_int32
     lib.zodi_number_density.argtypes = [_p, _p, _p, _i64, _p, _p, _p]
+    lib.zodi_host_device_pointer.restype = ct.c_int32
+    lib.zodi_host_device_pointer.argtypes = [_p, ct.POINTER(ct.c_void_p)]
     lib.zodi_adjoint_workspace.restype = ct.c_int64
     lib.zodi_adjoint_workspace.argtypes = [_p, _i64, ct.POINTER(Launch)]
     lib.zodi_adjoint.restype = ct.c_int32
@@ -111,3 +114,10 @@ def check(status: int, what: str) -> None:
 
 def host_ptr(arr: np.ndarray) -> int:
     return arr.ctypes.data
+
+
+def device_view(host_address: int) -> int:
+    """Device address of page-locked host memory (raises for anything else)."""
+    out = ct.c_void_p()
+    check(load().zodi_host_device_pointer(host_address, ct.byref(out)), "zodi_host_device_pointer")
+    return int(out.value)

@@ -23,6 +23,8 @@ HEADERS = [
     CSRC / "zodi_device.hpp",
     CSRC / "zodi_grad.hpp",
     CSRC / "zodi_math.hpp",
+    CSRC / "zodi_trig.hpp",
+    CSRC / "zodi_trig_coeffs.inc",
     INCLUDE / "zodi_hip.h",
 ]
 

@@ -113,8 +113,13 @@ __device__ __forceinline__ void adjoint_group(Grp<Ks...>, std::integer_sequence<
    ...);
   const double solar = cx.solar;
   // Phi enters d/d(albedo) when F != 0, and d/dF, d/dC_i when an albedo is non-zero.
+  // With every albedo 0 the C_i partials are multiplied by 0, and with C2 = C3 = 0
+  // Phi = N (C1 + 1) does not depend on the scattering angle: skip the angle.
   const bool need_phase =
       INTEGRAND == ZODI_INTEGRAND_KELSALL && (solar != 0.0 || (M->flags & 1) != 0);
+  const bool need_angle = need_phase && ((M->flags & 1) != 0 || M->shared[ZODI_S_C2] != 0.0 ||
+                                         M->shared[ZODI_S_C3] != 0.0);
+  const double phi_const = M->shared[ZODI_S_PHASE_NORM] * (M->shared[ZODI_S_C1] + 1.0);
 
   std::tuple<CompAcc<kind_nparams<kinds[J]>()>...> acc{};
   double sh[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};  // Kelsall: T0, delta, F, C1..C3; RRM: cal
@@ -137,8 +142,8 @@ __device__ __forceinline__ void adjoint_group(Grp<Ks...>, std::integer_sequence<
                                cx.tb.inv_step, dBdT);
     const double dT_dT0 = T * inv_T0;
     const double dT_ddelta = -T * logR;
-    PhaseGrad pg = {0.0, 0.0, 0.0, 0.0};
-    if (need_phase) {
+    PhaseGrad pg = {need_phase ? phi_const : 0.0, 0.0, 0.0, 0.0};
+    if (need_angle) {
       pg = phase_grad(y.ux, y.uy, y.uz, nd.X0, nd.X1, nd.X2, nd.inv_R, Mk->shared);
     }
     const double inv_R2 = nd.inv_R * nd.inv_R;
@@ -254,7 +259,8 @@ __device__ __forceinline__ void run_adjoint(Layout<INTEGRAND, Gs...>, const Ctx&
 
 // One wave per tile of 64 / SPLIT rays; SPLIT lanes share a ray's nodes.
 template <class LAYOUT, int CHUNK, int SPLIT>
-__global__ __launch_bounds__(kAdjBlock) void adjoint_layout(AdjArgs a) {
+__global__ __launch_bounds__(kAdjBlock) __attribute__((amdgpu_waves_per_eu(2, 2))) void
+adjoint_layout(AdjArgs a) {
   constexpr int RAYS = kWave / SPLIT;
   extern __shared__ double sRows[];  // kAdjWaves rows of n_params + 1
   __shared__ double sTab[4 * ZODI_N_TEMPS];
@@ -398,7 +404,7 @@ zodi_status validate_grad_layout(const zodi_model* m) {
   return ZODI_OK;
 }
 
-constexpr int kDefaultChunk = 1;
+constexpr int kDefaultChunk = 2;
 constexpr int kDefaultBlocksPerCu = 2;
 
 struct AdjPlan {

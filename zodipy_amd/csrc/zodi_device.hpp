@@ -14,6 +14,7 @@
 
 #include "zodi_hip.h"
 #include "zodi_math.hpp"
+#include "zodi_trig.hpp"
 
 namespace zodi {
 
@@ -176,8 +177,8 @@ __device__ __forceinline__ CompFrame comp_frame(ParamPtr P, uint32_t flags, doub
 
 // ---------------------------------------------------------------- densities
 // Cloud, zodipy/number_density.py:47-73:
-// n_0 R^-alpha exp_fast(-beta g^gamma), g = zeta^2/(2 mu) if zeta < mu else zeta - mu/2,
-// evaluated as n_0 exp_fast(-alpha log R - beta g^gamma).
+// n_0 R^-alpha exp(-beta g^gamma), g = zeta^2/(2 mu) if zeta < mu else zeta - mu/2,
+// evaluated as n_0 exp(-alpha log R - beta g^gamma).
 __device__ __forceinline__ double cloud_density(ParamPtr P, const CompFrame& f) {
   const double n0 = P[6], alpha = P[7], beta = P[8], gamma = P[9], mu = P[10];
   const double inv_2mu = P[13], half_mu = P[14];
@@ -195,7 +196,7 @@ __device__ __forceinline__ double pow20(double x) {
 }
 
 // Band, zodipy/number_density.py:76-110 (P[6] holds 3 n_0; the 1/v factor as in
-// the reference): 3 n_0/R exp_fast(-zd^6) (1 + zd^p / v) (1 - exp_fast(-(R/delta_r)^20)),
+// the reference): 3 n_0/R exp(-zd^6) (1 + zd^p / v) (1 - exp(-(R/delta_r)^20)),
 // zd = zeta / delta_zeta.
 __device__ __forceinline__ double band_density(ParamPtr P, uint32_t flags, const CompFrame& f) {
   const double three_n0 = P[6], p = P[8];
@@ -210,20 +211,20 @@ __device__ __forceinline__ double band_density(ParamPtr P, uint32_t flags, const
   return (three_n0 * f.inv_r) * exp_fast(-zd6) * term3 * term4;
 }
 
-// Ring, zodipy/number_density.py:113-139: n_0 exp_fast(-(R - R_r)^2/sigma_r^2 - |Z|/sigma_z).
+// Ring, zodipy/number_density.py:113-139: n_0 exp(-(R - R_r)^2/sigma_r^2 - |Z|/sigma_z).
 __device__ __forceinline__ double ring_density(ParamPtr P, const CompFrame& f) {
   const double n0 = P[6], Rr = P[7];
   const double d = f.r - Rr;
   return n0 * exp_fast(-(d * d) * P[13] - fabs(f.zc) * P[14]);
 }
 
-// Feature, zodipy/number_density.py:142-181: n_0 exp_fast(-((R - R_f)^2/sigma_r^2 +
+// Feature, zodipy/number_density.py:142-181: n_0 exp(-((R - R_f)^2/sigma_r^2 +
 // |Z|/sigma_z + dtheta^2/sigma_theta^2)), dtheta wrapped with a floor modulo;
 // `earth_lon` = atan2(E'_y, E'_x) with E' = X_earth - X_0.
 __device__ __forceinline__ double feature_density(ParamPtr P, const CompFrame& f,
                                                   double earth_lon) {
   const double n0 = P[6], Rf = P[7], th = P[10];
-  const double theta_comp = atan2(f.y, f.x) - earth_lon;
+  const double theta_comp = zodi_trig::atan2_fast(f.y, f.x) - earth_lon;
   double dth = theta_comp - th;
   dth = floor_mod_pos(dth + kPi, kTwoPi) - kPi;
   const double d = f.r - Rf;
@@ -313,14 +314,14 @@ __device__ __forceinline__ double density(int kind, ParamPtr P, uint32_t flags, 
 
 // Scattering phase function at the node (zodipy/scattering.py:11-50):
 // Theta = arccos(-clip(X_los . X_helio / (R_los R_helio), -1, 1)),
-// Phi = N (C1 + C2 Theta + exp_fast(C3 Theta)).
+// Phi = N (C1 + C2 Theta + exp(C3 Theta)).
 __device__ __forceinline__ double phase_function(double R_los, double ux, double uy, double uz,
                                                  double X0, double X1, double X2, double R,
                                                  ParamPtr S) {
   const double lx = R_los * ux, ly = R_los * uy, lz = R_los * uz;
   double ct = (lx * X0 + ly * X1 + lz * X2) / (R_los * R);
   ct = ct != ct ? ct : (ct < -1.0 ? -1.0 : (ct > 1.0 ? 1.0 : ct));
-  const double theta = acos(-ct);
+  const double theta = zodi_trig::acos_fast(-ct);
   return S[ZODI_S_PHASE_NORM] * (S[ZODI_S_C1] + S[ZODI_S_C2] * theta + exp_fast(S[ZODI_S_C3] * theta));
 }
 

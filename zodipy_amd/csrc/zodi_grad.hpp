@@ -146,7 +146,7 @@ __device__ __forceinline__ DensGrad<6> feature_grad(ParamPtr P, const CompFrame&
                                                     double earth_lon) {
   const double n0 = P[6], Rf = P[7], th = P[10];
   const double inv_sr2 = P[13], inv_sz = P[14], inv_sth2 = P[15], inv_sr = P[16], inv_sth = P[17];
-  const double theta_comp = atan2(f.y, f.x) - earth_lon;
+  const double theta_comp = zodi_trig::atan2_fast(f.y, f.x) - earth_lon;
   double dth = theta_comp - th;
   dth = floor_mod_pos(dth + kPi, kTwoPi) - kPi;
   const double d = f.r - Rf;
@@ -263,7 +263,7 @@ __device__ __forceinline__ PhaseGrad phase_grad(double ux, double uy, double uz,
                                                 double X1, double X2, double inv_R, ParamPtr S) {
   double ct = (ux * X0 + uy * X1 + uz * X2) * inv_R;
   ct = ct != ct ? ct : (ct < -1.0 ? -1.0 : (ct > 1.0 ? 1.0 : ct));
-  const double theta = acos(-ct);
+  const double theta = zodi_trig::acos_fast(-ct);
   const double C1 = S[ZODI_S_C1], C2 = S[ZODI_S_C2], C3 = S[ZODI_S_C3];
   const double N = S[ZODI_S_PHASE_NORM];
   const double e3 = exp_fast(C3 * theta);
