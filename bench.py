@@ -16,6 +16,8 @@ Also reported (not the headline value):
 * fwd_grad_over_fwd: chi^2 forward + gradient time over forward time, DIRBE
   25 um at nside=128; ``grad`` adds the batched gradient of a six-component
   Planck-2018-style model at 857 GHz over 1000 observers;
+* ingest: the pointing-ingestion kernels (HEALPix pixel centres and lon/lat to
+  ecliptic unit vectors), HBM-bound, with their GB/s;
 * cpu_baseline: the NumPy oracle with the reference's loop structure and its
   fork-pool split over host cores, one pool per observer call as the reference
   creates it, on a bounded prefix of the same observers.
@@ -256,6 +258,41 @@ def grad_cost(dev, reps: int = 10) -> dict:
     return out
 
 
+def ingest_cost(dev, n_rays: int, reps: int = 5) -> dict:
+    """Pointing ingestion kernels: HBM-bound conversions to ecliptic unit vectors.
+
+    pix2vec: every pixel of a nside=2048 HEALPix map (50 331 648 pixels) in the
+    galactic frame; lonlat: n_rays galactic longitude/latitude pairs (degrees).
+    Algorithmic bytes: 24 written per element, plus 16 read for lonlat.
+    """
+    import torch
+
+    from zodipy_amd import pointing
+
+    out = {}
+    nside = 2048
+    npix = 12 * nside * nside
+    pointing.pix2vec(nside, frame="galactic")
+    t = _event_time(lambda: pointing.pix2vec(nside, frame="galactic"), reps)
+    out["pix2vec_nside2048_galactic"] = {
+        "n": npix, "ms": round(t, 4), "GBs": round(24 * npix / (t / 1e3) / 1e9, 1),
+        "frac_of_hbm": round(24 * npix / (t / 1e3) / 1e9 / PEAK_HBM_GBS, 3)}
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(SEED + 7)
+    lon = torch.rand(n_rays, generator=gen, dtype=torch.float64, device=dev) * 360.0
+    lat = torch.rand(n_rays, generator=gen, dtype=torch.float64, device=dev) * 180.0 - 90.0
+    pointing.lonlat_to_vectors(lon, lat, frame="galactic", degrees=True)
+    t = _event_time(lambda: pointing.lonlat_to_vectors(lon, lat, frame="galactic",
+                                                       degrees=True), reps)
+    out["lonlat_galactic"] = {
+        "n": n_rays, "ms": round(t, 4), "GBs": round(40 * n_rays / (t / 1e3) / 1e9, 1),
+        "frac_of_hbm": round(40 * n_rays / (t / 1e3) / 1e9 / PEAK_HBM_GBS, 3)}
+    log(f"ingestion: pix2vec nside 2048 {out['pix2vec_nside2048_galactic']['ms']:.3f} ms "
+        f"({out['pix2vec_nside2048_galactic']['GBs']} GB/s), lonlat {n_rays} "
+        f"{out['lonlat_galactic']['ms']:.3f} ms ({out['lonlat_galactic']['GBs']} GB/s)")
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -445,6 +482,8 @@ def main() -> int:
         cpu = cpu_baseline(u_host, seg, obs, args.cpu_seconds)
         log(f"cpu baseline: {cpu['value']:.3e} samples/s ({cpu['sample']})")
 
+    ingest = None if args.profile else ingest_cost(dev, n_rays)
+
     grad_ratio = None
     if not args.no_grad:
         grad_ratio = grad_cost(dev)
@@ -479,6 +518,7 @@ def main() -> int:
         "batched_vs_plain": (value / plain["samples_per_s"]) if plain else None,
         "batched_e2e_vs_plain": (batched_e2e["samples_per_s"] / plain["samples_per_s"])
         if plain and batched_e2e else None,
+        "ingest": ingest,
         "fwd_grad_over_fwd": grad_ratio["ratio"] if grad_ratio else None,
         "grad": grad_ratio,
     }

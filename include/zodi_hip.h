@@ -16,6 +16,8 @@
  *   zodi_number_density   zodipy/number_density.py:47-420 (component densities at
  *                         heliocentric points; the kernel behind grid_number_density,
  *                         zodipy/number_density.py:482-536)
+ *   zodi_pix2vec,         zodipy/model.py:247-251 (the frame rotation and cartesian
+ *   zodi_lonlat_to_vec    conversion of the pointings; HEALPix pixel centres)
  *   zodi_adjoint          no reference counterpart: vector-Jacobian product of the
  *                         forward map with respect to the model parameter vector
  *                         (the order of zodipy/zodiacal_light_model.py:38-55's
@@ -221,6 +223,26 @@ zodi_status zodi_line_of_sight(const zodi_model* model_host, const zodi_model* m
 zodi_status zodi_number_density(const zodi_model* model_host, const zodi_model* model_dev,
                                 const double* X, int64_t n_points,
                                 const double* earth, double* out, void* stream);
+
+/*
+ * Pointing ingestion: ecliptic unit vectors out[3][n] (device memory) for
+ * zodi_forward, replacing skycoord.transform_to(BarycentricMeanEcliptic) and
+ * .cartesian (zodipy/model.py:247-251). rot is a 3x3 row-major rotation to the
+ * mean ecliptic and equinox of J2000 in host memory, or NULL for none.
+ *
+ * zodi_pix2vec — centres of HEALPix RING pixels of resolution nside (what the
+ * reference's HEALPix map examples build with healpy's pix2ang,
+ * docs/examples/healpy_map.py:16): pix[n] (device memory), or, with pix NULL,
+ * the pixels first .. first + n - 1. An index outside [0, 12 nside^2) gives NaN.
+ *
+ * zodi_lonlat_to_vec — (cos b cos l, cos b sin l, sin b) for longitudes lon[n]
+ * and latitudes lat[n] (device memory), in degrees when degrees != 0, else radians.
+ */
+zodi_status zodi_pix2vec(int64_t nside, const int64_t* pix, int64_t first, int64_t n,
+                         const double* rot, double* out, void* stream);
+
+zodi_status zodi_lonlat_to_vec(const double* lon, const double* lat, int64_t n, int32_t degrees,
+                               const double* rot, double* out, void* stream);
 
 /*
  * zodi_host_device_pointer — the device address of page-locked, mapped host

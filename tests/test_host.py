@@ -94,6 +94,12 @@ def test_host_validation_errors():
                             dummy, 0, None, None) == 1
     assert lib.zodi_forward(blk.ctypes.data, dummy, dummy, 1, dummy, 1, dummy, dummy, 0, dummy,
                             dummy, 9, None, None) == 1
+    dst = ct.c_void_p(0x3000)
+    assert lib.zodi_pix2vec(0, None, 0, 10, None, dst, None) == 1            # nside < 1
+    assert lib.zodi_pix2vec(4, None, 150, 60, None, dst, None) == 1          # past 192 pixels
+    assert lib.zodi_pix2vec(4, None, 0, 10, None, None, None) == 1           # no output
+    assert lib.zodi_lonlat_to_vec(None, None, 5, 0, None, dst, None) == 1
+    assert lib.zodi_lonlat_to_vec(None, None, 0, 0, None, None, None) == 0   # nothing to do
     out = ct.c_void_p()
     pageable = np.zeros(16)
     assert lib.zodi_host_device_pointer(pageable.ctypes.data, ct.byref(out)) == 1

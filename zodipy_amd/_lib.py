@@ -27,6 +27,8 @@ EXPORTED_SYMBOLS = (
     "zodi_forward",
     "zodi_line_of_sight",
     "zodi_number_density",
+    "zodi_pix2vec",
+    "zodi_lonlat_to_vec",
     "zodi_host_device_pointer",
     "zodi_adjoint_workspace",
     "zodi_adjoint",
@@ -65,6 +67,10 @@ def _bind(lib: ct.CDLL) -> None:
     lib.zodi_line_of_sight.argtypes = [_p, _p, _p, _i64, _p, _i32, _p, _i32, _p, _p, _p, _p]
     lib.zodi_number_density.restype = ct.c_int32
     lib.zodi_number_density.argtypes = [_p, _p, _p, _i64, _p, _p, _p]
+    lib.zodi_pix2vec.restype = ct.c_int32
+    lib.zodi_pix2vec.argtypes = [_i64, _p, _i64, _i64, _p, _p, _p]
+    lib.zodi_lonlat_to_vec.restype = ct.c_int32
+    lib.zodi_lonlat_to_vec.argtypes = [_p, _p, _i64, _i32, _p, _p, _p]
     lib.zodi_host_device_pointer.restype = ct.c_int32
     lib.zodi_host_device_pointer.argtypes = [_p, ct.POINTER(ct.c_void_p)]
     lib.zodi_adjoint_workspace.restype = ct.c_int64

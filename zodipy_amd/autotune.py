@@ -11,6 +11,7 @@ from __future__ import annotations
 
 import json
 import os
+import sys
 from pathlib import Path
 
 import numpy as np
@@ -129,7 +130,7 @@ def tune(dm: DeviceModel, u: torch.Tensor, seg, n_seg: int, obs, earth, beyond, 
         times[f"{b},{lanes},{per_cu}"] = time_launch(fn)
         if verbose:
             print(f"  tune {b:4d} threads, {lanes} lanes/ray, {per_cu} blocks/CU: "
-                  f"{times[f'{b},{lanes},{per_cu}']:.3f} ms", flush=True)
+                  f"{times[f'{b},{lanes},{per_cu}']:.3f} ms", file=sys.stderr, flush=True)
     best = min(times, key=times.get)
     b, lanes, per_cu = (int(v) for v in best.split(","))
     if use_cache:

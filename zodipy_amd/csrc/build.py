@@ -17,7 +17,7 @@ PKG = CSRC.parent
 ROOT = PKG.parent
 INCLUDE = ROOT / "include"
 LIB = PKG / "libzodi_hip.so"
-SOURCES = [CSRC / "zodi_forward.hip", CSRC / "zodi_adjoint.hip"]
+SOURCES = [CSRC / "zodi_forward.hip", CSRC / "zodi_adjoint.hip", CSRC / "zodi_pointing.hip"]
 HEADERS = [
     CSRC / "zodi_common.hpp",
     CSRC / "zodi_device.hpp",
