@@ -262,8 +262,10 @@ def ingest_cost(dev, n_rays: int, reps: int = 5) -> dict:
     """Pointing ingestion kernels: HBM-bound conversions to ecliptic unit vectors.
 
     pix2vec: every pixel of a nside=2048 HEALPix map (50 331 648 pixels) in the
-    galactic frame; lonlat: n_rays galactic longitude/latitude pairs (degrees).
-    Algorithmic bytes: 24 written per element, plus 16 read for lonlat.
+    galactic frame; lonlat: n_rays galactic longitude/latitude pairs (degrees);
+    track positions: n_rays observation times on a one-year hourly spline.
+    Algorithmic bytes: 24 written per element, plus 16 read for lonlat and 8 for
+    the track times.
     """
     import torch
 
@@ -287,9 +289,24 @@ def ingest_cost(dev, n_rays: int, reps: int = 5) -> dict:
     out["lonlat_galactic"] = {
         "n": n_rays, "ms": round(t, 4), "GBs": round(40 * n_rays / (t / 1e3) / 1e9, 1),
         "frac_of_hbm": round(40 * n_rays / (t / 1e3) / 1e9 / PEAK_HBM_GBS, 3)}
+    from zodipy_amd.ephemeris import BodyTrack
+
+    grid = np.arange(59580.0, 59580.0 + 365.0, 1.0 / 24.0)  # one year, hourly
+    ph = 2.0 * np.pi * (grid - 59580.0) / 365.25
+    track = BodyTrack(grid, np.stack([np.cos(ph), np.sin(ph), np.zeros_like(ph)]))
+    mjd = (torch.rand(n_rays, generator=gen, dtype=torch.float64, device=dev) * 364.0
+           + 59580.0)
+    track.on_device(mjd, dev)
+    t = _event_time(lambda: track.on_device(mjd, dev), reps)
+    out["spline_positions_one_year_hourly"] = {
+        "n": n_rays, "knots": int(grid.size), "ms": round(t, 4),
+        "GBs": round(32 * n_rays / (t / 1e3) / 1e9, 1),
+        "frac_of_hbm": round(32 * n_rays / (t / 1e3) / 1e9 / PEAK_HBM_GBS, 3)}
     log(f"ingestion: pix2vec nside 2048 {out['pix2vec_nside2048_galactic']['ms']:.3f} ms "
         f"({out['pix2vec_nside2048_galactic']['GBs']} GB/s), lonlat {n_rays} "
-        f"{out['lonlat_galactic']['ms']:.3f} ms ({out['lonlat_galactic']['GBs']} GB/s)")
+        f"{out['lonlat_galactic']['ms']:.3f} ms ({out['lonlat_galactic']['GBs']} GB/s), "
+        f"track positions {out['spline_positions_one_year_hourly']['ms']:.3f} ms "
+        f"({out['spline_positions_one_year_hourly']['GBs']} GB/s)")
     return out
 
 

@@ -18,6 +18,7 @@
  *                         zodipy/number_density.py:482-536)
  *   zodi_pix2vec,         zodipy/model.py:247-251 (the frame rotation and cartesian
  *   zodi_lonlat_to_vec    conversion of the pointings; HEALPix pixel centres)
+ *   zodi_spline_positions zodipy/bodies.py:22-35 (time-ordered body positions)
  *   zodi_adjoint          no reference counterpart: vector-Jacobian product of the
  *                         forward map with respect to the model parameter vector
  *                         (the order of zodipy/zodiacal_light_model.py:38-55's
@@ -243,6 +244,17 @@ zodi_status zodi_pix2vec(int64_t nside, const int64_t* pix, int64_t first, int64
 
 zodi_status zodi_lonlat_to_vec(const double* lon, const double* lat, int64_t n, int32_t degrees,
                                const double* rot, double* out, void* stream);
+
+/*
+ * zodi_spline_positions — heliocentric positions out[3][n] at times t[n] along a
+ * cubic spline in piecewise-polynomial form (breakpoints x[m] ascending,
+ * coefficients c[4][m-1][3] with c[0] multiplying (t - x_k)^3, all in device
+ * memory): the per-ray observer and Earth positions of time-ordered data, which
+ * the reference interpolates on an hourly grid with scipy's CubicSpline
+ * (zodipy/bodies.py:22-35). Intervals follow scipy (extrapolating at both ends).
+ */
+zodi_status zodi_spline_positions(const double* x, int64_t m, const double* c, const double* t,
+                                  int64_t n, double* out, void* stream);
 
 /*
  * zodi_host_device_pointer — the device address of page-locked, mapped host

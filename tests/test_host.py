@@ -100,6 +100,8 @@ def test_host_validation_errors():
     assert lib.zodi_pix2vec(4, None, 0, 10, None, None, None) == 1           # no output
     assert lib.zodi_lonlat_to_vec(None, None, 5, 0, None, dst, None) == 1
     assert lib.zodi_lonlat_to_vec(None, None, 0, 0, None, None, None) == 0   # nothing to do
+    assert lib.zodi_spline_positions(dst, 1, dst, dst, 5, dst, None) == 1    # one breakpoint
+    assert lib.zodi_spline_positions(dst, 4, None, dst, 5, dst, None) == 1
     out = ct.c_void_p()
     pageable = np.zeros(16)
     assert lib.zodi_host_device_pointer(pageable.ctypes.data, ct.byref(out)) == 1
@@ -320,3 +322,20 @@ def test_bench_refuses_more_than_one_gpu():
                        capture_output=True, text=True, timeout=120)
     assert r.returncode != 0
     assert "one GPU" in r.stderr
+
+
+def test_body_track_host_and_validation():
+    from zodipy_amd.ephemeris import BodyTrack, arrange_obstimes
+
+    grid = arrange_obstimes(59000.0, 59002.0)
+    assert np.allclose(np.diff(grid), 1 / 24)
+    pos = np.stack([np.cos(grid / 58.1), np.sin(grid / 58.1), 1e-4 * np.cos(grid)])
+    tr = BodyTrack(grid, pos)
+    # the spline interpolates its knots (the last one through the last interval)
+    np.testing.assert_array_equal(tr(grid[:-1]), pos[:, :-1])
+    np.testing.assert_allclose(tr(grid[-1]), pos[:, -1], rtol=1e-14, atol=1e-17)
+    assert tr.c.shape == (4, grid.size - 1, 3)
+    with pytest.raises(ValueError):
+        BodyTrack(grid[:1], pos[:, :1])
+    with pytest.raises(ValueError):
+        BodyTrack(grid[::-1], pos)

@@ -4,7 +4,9 @@
 // * zodi_pix2vec: centres of HEALPix RING-scheme pixels (Gorski et al. 2005,
 //   ApJ 622, 759, eqs. 4-5), the pointings of the reference's HEALPix map
 //   examples (docs/examples/healpy_map.py:16 builds them with healpy's pix2ang);
-// * zodi_lonlat_to_vec: longitude/latitude pairs, as SkyCoord holds them.
+// * zodi_lonlat_to_vec: longitude/latitude pairs, as SkyCoord holds them;
+// * zodi_spline_positions: observer/Earth positions per ray for time-ordered
+//   data, from the cubic spline of an hourly ephemeris grid.
 //
 // Both apply a 3x3 frame rotation to the mean ecliptic and equinox of J2000,
 // the work of skycoord.transform_to(BarycentricMeanEcliptic) and .cartesian in
@@ -113,6 +115,67 @@ __global__ __launch_bounds__(256) void lonlat_kernel(const double* __restrict__ 
   }
 }
 
+// Positions along a tabulated track: the cubic spline of an hourly ephemeris
+// grid (zodipy/bodies.py:22-35 builds scipy's CubicSpline over it), in scipy's
+// piecewise-polynomial form. The interval is scipy's (the last breakpoint at or
+// below t, clipped to the first and last intervals, which then extrapolate) and
+// the polynomial is summed in increasing powers of t - x_k as scipy does.
+// The interval is first guessed as if the grid were uniform (the hourly grids
+// of time-ordered data are), checked against its two breakpoints, and only
+// searched by bisection when the guess misses.
+__global__ __launch_bounds__(256) void spline_kernel(const double* __restrict__ x, int64_t m,
+                                                     const double* __restrict__ c,
+                                                     const double* __restrict__ t, int64_t n,
+                                                     double* __restrict__ out) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  const int64_t ni = m - 1;  // intervals
+  const double x0 = x[0];
+  const double xl = x[m - 1];
+  // Reached only with x0 <= t < xl, so the spacing is positive there; a guess
+  // outside [0, ni - 1] (an infinite or NaN quotient) is clipped below.
+  const double inv_step = (double)ni / (xl - x0);
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const double ti = t[i];
+    int64_t k;
+    if (!(ti >= x0)) {
+      k = 0;  // below the grid, or NaN (which then propagates)
+    } else if (ti >= xl) {
+      k = ni - 1;
+    } else {
+      const double g = (ti - x0) * inv_step;
+      k = (g >= 0.0 && g < (double)(ni - 1)) ? (int64_t)g : ni - 1;
+      if (!(x[k] <= ti && ti < x[k + 1])) {
+        int64_t lo = 0, hi = m - 1;  // x[lo] <= ti < x[hi]
+        while (hi - lo > 1) {
+          const int64_t mid = (lo + hi) >> 1;
+          if (x[mid] <= ti) {
+            lo = mid;
+          } else {
+            hi = mid;
+          }
+        }
+        k = lo;
+      }
+    }
+    {
+      // c is [4][ni][3]; c[0] multiplies s^3. Separately rounded products and sums
+      // (no fused multiply-add), term by term in increasing powers, as scipy does.
+#pragma clang fp contract(off)
+      const double s = ti - x[k];
+      const double s2 = s * s;
+      const double s3 = s2 * s;
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        double res = c[(3 * ni + k) * 3 + d];
+        res = res + c[(2 * ni + k) * 3 + d] * s;
+        res = res + c[(1 * ni + k) * 3 + d] * s2;
+        res = res + c[k * 3 + d] * s3;
+        out[d * n + i] = res;
+      }
+    }
+  }
+}
+
 namespace {
 
 int64_t elementwise_grid(int64_t n) {
@@ -164,6 +227,19 @@ zodi_status zodi_lonlat_to_vec(const double* lon, const double* lat, int64_t n, 
   hipLaunchKernelGGL(lonlat_kernel, dim3((unsigned)elementwise_grid(n)), dim3(256), 0,
                      (hipStream_t)stream, lon, lat, n, scale, R, identity, out);
   return check_hip(hipGetLastError(), "lonlat launch");
+}
+
+zodi_status zodi_spline_positions(const double* x, int64_t m, const double* c, const double* t,
+                                  int64_t n, double* out, void* stream) {
+  if (m < 2) return fail(ZODI_ERR_ARG, "a track needs at least 2 breakpoints, got %lld",
+                         (long long)m);
+  if (n < 0) return fail(ZODI_ERR_ARG, "n < 0");
+  if (n == 0) return ZODI_OK;
+  if (x == nullptr || c == nullptr || t == nullptr || out == nullptr)
+    return fail(ZODI_ERR_ARG, "null pointer");
+  hipLaunchKernelGGL(spline_kernel, dim3((unsigned)elementwise_grid(n)), dim3(256), 0,
+                     (hipStream_t)stream, x, m, c, t, n, out);
+  return check_hip(hipGetLastError(), "spline launch");
 }
 
 }  // extern "C"

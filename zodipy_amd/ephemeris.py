@@ -78,3 +78,67 @@ def interp_body_position(body: str, mjd: np.ndarray, grid_mjd: np.ndarray,
     """Cubic-spline interpolation of an hourly ephemeris grid (zodipy/bodies.py:22-35)."""
     pos = body_position(body, grid_mjd, ephemeris)
     return interpolate.CubicSpline(grid_mjd, pos, axis=-1)(mjd)
+
+
+class BodyTrack:
+    """A body's heliocentric ecliptic positions tabulated on a time grid.
+
+    For time-ordered data the reference interpolates an hourly ephemeris grid
+    with ``scipy.interpolate.CubicSpline(grid_mjd, positions, axis=-1)``
+    (zodipy/bodies.py:22-35). A ``BodyTrack`` holds that spline (the grid and
+    positions come from the caller, e.g. exported from an ephemeris; no network
+    or astropy needed) and evaluates it either on the host (``__call__``) or per
+    ray on the GPU (``on_device``, kernel ``zodi_spline_positions``), where it
+    rounds bit for bit like scipy's evaluation.
+
+    Args:
+        mjd_grid: (m,) increasing Modified Julian Dates, m >= 2.
+        positions: (3, m) heliocentric ecliptic positions [AU].
+    """
+
+    def __init__(self, mjd_grid, positions) -> None:
+        x = np.asarray(mjd_grid, dtype=np.float64).ravel()
+        pos = np.asarray(positions, dtype=np.float64)
+        if x.size < 2 or pos.shape != (3, x.size):
+            msg = f"need >= 2 grid times and positions of shape (3, {x.size}), got {pos.shape}"
+            raise ValueError(msg)
+        if np.any(np.diff(x) <= 0):
+            msg = "grid times must be strictly increasing"
+            raise ValueError(msg)
+        self.x = x
+        self.spline = interpolate.CubicSpline(x, pos, axis=-1)
+        self.c = np.ascontiguousarray(self.spline.c, dtype=np.float64)  # (4, m - 1, 3)
+        self._dev = {}
+
+    @classmethod
+    def hourly(cls, body_position_fn, t0_mjd: float, t1_mjd: float) -> "BodyTrack":
+        """Track on the reference's hourly grid over [t0, t1] from ``fn(mjd) -> (3, m)``."""
+        grid = arrange_obstimes(t0_mjd, t1_mjd)
+        return cls(grid, body_position_fn(grid))
+
+    def __call__(self, mjd) -> np.ndarray:
+        return self.spline(np.asarray(mjd, dtype=np.float64))
+
+    def on_device(self, mjd, device="cuda"):
+        """Positions (3, n) at times ``mjd`` (array or tensor), evaluated on the GPU."""
+        import torch
+
+        from zodipy_amd import _lib
+        from zodipy_amd.runtime import _require_cuda, _stream
+
+        dev = _require_cuda(device)
+        key = str(dev)
+        if key not in self._dev:
+            self._dev[key] = (torch.as_tensor(self.x, device=dev),
+                              torch.as_tensor(self.c, device=dev))
+        x_t, c_t = self._dev[key]
+        if isinstance(mjd, torch.Tensor):
+            t = mjd.to(device=dev, dtype=torch.float64).reshape(-1).contiguous()
+        else:
+            t = torch.as_tensor(np.asarray(mjd, dtype=np.float64).ravel(), device=dev)
+        out = torch.empty((3, t.numel()), dtype=torch.float64, device=dev)
+        st = _lib.load().zodi_spline_positions(x_t.data_ptr(), self.x.size, c_t.data_ptr(),
+                                               t.data_ptr(), t.numel(), out.data_ptr(),
+                                               _stream())
+        _lib.check(st, "zodi_spline_positions")
+        return out

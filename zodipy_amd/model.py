@@ -119,7 +119,9 @@ class Model:
         """Return the zodiacal emission [MJy/sr] towards every coordinate of ``skycoord``.
 
         Follows zodipy/model.py:119-203 (validation, instantaneous vs
-        time-ordered observers, component sum in model order).
+        time-ordered observers, component sum in model order). ``earthpos`` (and
+        ``obspos``) may also be an ``ephemeris.BodyTrack``: for time-ordered data
+        its spline is then evaluated per ray on the GPU.
         """
         try:
             if skycoord.obstime is None:
@@ -130,7 +132,7 @@ class Model:
             raise TypeError(msg) from error
 
         obspos_isstr = isinstance(obspos, str)
-        if not obspos_isstr:
+        if not obspos_isstr and not isinstance(obspos, ephemeris.BodyTrack):
             if not isinstance(obspos, units.Quantity):
                 msg = "The observer position is not a string or a Quantity."
                 raise TypeError(msg)
@@ -163,6 +165,11 @@ class Model:
         return out
 
     def _earth_position(self, skycoord, earthpos, instantaneous: bool) -> np.ndarray:
+        if isinstance(earthpos, ephemeris.BodyTrack):
+            mjd = np.asarray(_mjd(skycoord.obstime), dtype=np.float64)
+            if instantaneous:
+                return earthpos(mjd.reshape(())).reshape(3)
+            return earthpos.on_device(mjd, self._device)
         if earthpos is not None:
             e = earthpos.to_value(units.AU) if isinstance(earthpos, units.Quantity) else earthpos
             e = np.asarray(e, dtype=np.float64)
@@ -178,8 +185,17 @@ class Model:
         return ephemeris.interp_body_position("earth", mjd, grid, self._ephemeris)
 
     def _observer_position(self, skycoord, obspos, earth_xyz, instantaneous: bool) -> np.ndarray:
+        if isinstance(obspos, ephemeris.BodyTrack):
+            mjd = np.asarray(_mjd(skycoord.obstime), dtype=np.float64)
+            if instantaneous:
+                return obspos(mjd.reshape(())).reshape(3)
+            return obspos.on_device(mjd, self._device)
         if isinstance(obspos, str):
             if obspos == "semb-l2":
+                if isinstance(earth_xyz, torch.Tensor):
+                    # the norm over the whole array, as zodipy/bodies.py:47 takes it
+                    dist = torch.linalg.vector_norm(earth_xyz)
+                    return earth_xyz / dist * (dist + ephemeris.MEAN_DIST_TO_L2)
                 return ephemeris.get_semb_l2_pos(earth_xyz)
             if obspos == "earth":
                 return earth_xyz
@@ -216,8 +232,11 @@ class Model:
         ``return_comps`` else (N,), as a ``Quantity`` in MJy/sr, or as a device
         tensor with ``as_tensor=True``.
         """
-        obs = np.asarray(obspos, dtype=np.float64)
-        earth = np.asarray(earthpos, dtype=np.float64)
+        on_device = isinstance(obspos, torch.Tensor) and obspos.is_cuda
+        obs = obspos if on_device else np.asarray(obspos, dtype=np.float64)
+        earth = earthpos if isinstance(earthpos, torch.Tensor) and earthpos.is_cuda else \
+            np.asarray(earthpos.cpu() if isinstance(earthpos, torch.Tensor) else earthpos,
+                       dtype=np.float64)
         if per_ray is None:
             per_ray = obs.ndim == 2 and obs.shape[1] > 1
         if isinstance(unit_vectors, torch.Tensor):
@@ -227,7 +246,11 @@ class Model:
             if u.ndim == 1:
                 u = u.reshape(3, 1)
         n = u.shape[-1]
-        if per_ray:
+        if per_ray and on_device:
+            if not (isinstance(earth, torch.Tensor) and earth.is_cuda):
+                earth = torch.as_tensor(np.broadcast_to(np.asarray(earth).reshape(3, -1),
+                                                        (3, n)).copy(), device=obs.device)
+        elif per_ray:
             obs = np.broadcast_to(obs.reshape(3, -1), (3, n))
             earth = np.broadcast_to(earth.reshape(3, -1), (3, n))
         out_mode = OUT_COMPS if return_comps else OUT_SUM

@@ -71,8 +71,27 @@ class DeviceModel:
     def _ptrs(self):
         return _lib.host_ptr(self.host_block), self.dev_block.data_ptr()
 
+    def _device_per_ray(self, obs_xyz: torch.Tensor, earth_xyz: torch.Tensor):
+        """Per-ray observers already on the GPU: the cutoff bits come from the largest
+        observer distance, reduced on the device (one scalar returns to the host)."""
+        obs = obs_xyz.to(device=self.device, dtype=torch.float64).reshape(3, -1).contiguous()
+        earth = earth_xyz.to(device=self.device, dtype=torch.float64).reshape(3, -1).contiguous()
+        if earth.shape != obs.shape:
+            msg = f"earth positions {tuple(earth.shape)} do not match observers {tuple(obs.shape)}"
+            raise ValueError(msg)
+        # r = sqrt(x^2 + y^2 + z^2) as zodipy/line_of_sight.py:71 forms it; NaN distances
+        # compare false, as in (r_obs > cutoff).any()
+        r = torch.sqrt(obs[0] * obs[0] + obs[1] * obs[1] + obs[2] * obs[2])
+        r_max = float(torch.nan_to_num(r, nan=-1.0).max()) if r.numel() else -1.0
+        beyond = self.packed.beyond_bits(np.array([[r_max], [0.0], [0.0]]), any_over_all=True)
+        beyond_t = torch.from_numpy(beyond.view(np.int32).copy()).to(self.device)
+        return None, 0, obs, earth, beyond_t, OBS_PER_RAY
+
     def prepare_observers(self, obs_xyz, earth_xyz, seg_offsets=None, per_ray: bool = False):
         """Upload observer data; returns (seg, n_seg, obs, earth, beyond, mode)."""
+        if per_ray and isinstance(obs_xyz, torch.Tensor) and obs_xyz.is_cuda and \
+                isinstance(earth_xyz, torch.Tensor) and earth_xyz.is_cuda:
+            return self._device_per_ray(obs_xyz, earth_xyz)
         obs_np = np.asarray(obs_xyz.cpu() if isinstance(obs_xyz, torch.Tensor) else obs_xyz,
                             dtype=np.float64).reshape(3, -1)
         earth_np = np.asarray(
@@ -139,6 +158,16 @@ class DeviceModel:
         """
         u = _f64_rows(u, 3, "unit vectors", self.device)
         n_rays = u.shape[1]
+        if per_ray and isinstance(obs_xyz, torch.Tensor) and obs_xyz.is_cuda:
+            if obs_xyz.reshape(3, -1).shape[1] != n_rays:
+                msg = f"per-ray observers must be (3, {n_rays}), got {tuple(obs_xyz.shape)}"
+                raise ValueError(msg)
+            seg, n_seg, obs, earth, beyond, mode = self.prepare_observers(
+                obs_xyz, earth_xyz, None, True)
+            shape = (self.n_comps, n_rays) if out_mode == OUT_COMPS else (n_rays,)
+            out = torch.empty(shape, dtype=torch.float64, device=self.device)
+            return self.forward_raw(u, seg, n_seg, obs, earth, beyond, mode, out, out_mode,
+                                    launch)
         obs_np = np.asarray(obs_xyz.cpu() if isinstance(obs_xyz, torch.Tensor) else obs_xyz,
                             dtype=np.float64).reshape(3, -1)
         if not per_ray and seg_offsets is None:
