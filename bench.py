@@ -296,17 +296,23 @@ def ingest_cost(dev, n_rays: int, reps: int = 5) -> dict:
     track = BodyTrack(grid, np.stack([np.cos(ph), np.sin(ph), np.zeros_like(ph)]))
     mjd = (torch.rand(n_rays, generator=gen, dtype=torch.float64, device=dev) * 364.0
            + 59580.0)
-    track.on_device(mjd, dev)
-    t = _event_time(lambda: track.on_device(mjd, dev), reps)
-    out["spline_positions_one_year_hourly"] = {
-        "n": n_rays, "knots": int(grid.size), "ms": round(t, 4),
-        "GBs": round(32 * n_rays / (t / 1e3) / 1e9, 1),
-        "frac_of_hbm": round(32 * n_rays / (t / 1e3) / 1e9 / PEAK_HBM_GBS, 3)}
+    mjd_sorted = torch.sort(mjd).values  # time-ordered data: a scan in time
+    for key, times in (("spline_positions_one_year_hourly", mjd),
+                       ("spline_positions_one_year_hourly_time_ordered", mjd_sorted)):
+        track.on_device(times, dev)
+        t = _event_time(lambda: track.on_device(times, dev), reps)
+        out[key] = {
+            "n": n_rays, "knots": int(grid.size), "ms": round(t, 4),
+            "times": "random" if times is mjd else "sorted",
+            "GBs": round(32 * n_rays / (t / 1e3) / 1e9, 1),
+            "frac_of_hbm": round(32 * n_rays / (t / 1e3) / 1e9 / PEAK_HBM_GBS, 3)}
     log(f"ingestion: pix2vec nside 2048 {out['pix2vec_nside2048_galactic']['ms']:.3f} ms "
         f"({out['pix2vec_nside2048_galactic']['GBs']} GB/s), lonlat {n_rays} "
         f"{out['lonlat_galactic']['ms']:.3f} ms ({out['lonlat_galactic']['GBs']} GB/s), "
         f"track positions {out['spline_positions_one_year_hourly']['ms']:.3f} ms "
-        f"({out['spline_positions_one_year_hourly']['GBs']} GB/s)")
+        f"({out['spline_positions_one_year_hourly']['GBs']} GB/s) random times, "
+        f"{out['spline_positions_one_year_hourly_time_ordered']['ms']:.3f} ms "
+        f"({out['spline_positions_one_year_hourly_time_ordered']['GBs']} GB/s) sorted")
     return out
 
 

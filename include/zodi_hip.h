@@ -247,13 +247,16 @@ zodi_status zodi_lonlat_to_vec(const double* lon, const double* lat, int64_t n, 
 
 /*
  * zodi_spline_positions — heliocentric positions out[3][n] at times t[n] along a
- * cubic spline in piecewise-polynomial form (breakpoints x[m] ascending,
- * coefficients c[4][m-1][3] with c[0] multiplying (t - x_k)^3, all in device
- * memory): the per-ray observer and Earth positions of time-ordered data, which
- * the reference interpolates on an hourly grid with scipy's CubicSpline
- * (zodipy/bodies.py:22-35). Intervals follow scipy (extrapolating at both ends).
+ * cubic spline: the per-ray observer and Earth positions of time-ordered data,
+ * which the reference interpolates on an hourly grid with scipy's CubicSpline
+ * (zodipy/bodies.py:22-35). The spline is n_intervals records of 16 doubles in
+ * device memory (16-byte aligned), one per interval k of ascending breakpoints:
+ *   [0] x_k, [1..3] constant, [4..6] linear, [7..9] quadratic, [10..12] cubic
+ *   coefficients of (x, y, z) in powers of (t - x_k), [13] x_{k+1}, [14..15] 0
+ * (scipy's PPoly c[3-j][k][d] in slot 1 + 3 j + d). Intervals follow scipy
+ * (extrapolating at both ends) and the sums round as scipy's do.
  */
-zodi_status zodi_spline_positions(const double* x, int64_t m, const double* c, const double* t,
+zodi_status zodi_spline_positions(const double* records, int64_t n_intervals, const double* t,
                                   int64_t n, double* out, void* stream);
 
 /*

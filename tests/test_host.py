@@ -100,8 +100,9 @@ def test_host_validation_errors():
     assert lib.zodi_pix2vec(4, None, 0, 10, None, None, None) == 1           # no output
     assert lib.zodi_lonlat_to_vec(None, None, 5, 0, None, dst, None) == 1
     assert lib.zodi_lonlat_to_vec(None, None, 0, 0, None, None, None) == 0   # nothing to do
-    assert lib.zodi_spline_positions(dst, 1, dst, dst, 5, dst, None) == 1    # one breakpoint
-    assert lib.zodi_spline_positions(dst, 4, None, dst, 5, dst, None) == 1
+    assert lib.zodi_spline_positions(dst, 0, dst, 5, dst, None) == 1         # no interval
+    assert lib.zodi_spline_positions(dst, 4, None, 5, dst, None) == 1
+    assert lib.zodi_spline_positions(ct.c_void_p(0x3008), 4, dst, 5, dst, None) == 1  # align
     out = ct.c_void_p()
     pageable = np.zeros(16)
     assert lib.zodi_host_device_pointer(pageable.ctypes.data, ct.byref(out)) == 1
@@ -335,6 +336,13 @@ def test_body_track_host_and_validation():
     np.testing.assert_array_equal(tr(grid[:-1]), pos[:, :-1])
     np.testing.assert_allclose(tr(grid[-1]), pos[:, -1], rtol=1e-14, atol=1e-17)
     assert tr.c.shape == (4, grid.size - 1, 3)
+    rec = tr.records
+    assert rec.shape == (grid.size - 1, 16)
+    np.testing.assert_array_equal(rec[:, 0], grid[:-1])
+    np.testing.assert_array_equal(rec[:, 13], grid[1:])
+    np.testing.assert_array_equal(rec[:, 1:4], tr.c[3])   # constant term
+    np.testing.assert_array_equal(rec[:, 10:13], tr.c[0])  # cubic term
+    assert not rec[:, 14:].any()
     with pytest.raises(ValueError):
         BodyTrack(grid[:1], pos[:, :1])
     with pytest.raises(ValueError):

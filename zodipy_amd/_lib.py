@@ -73,7 +73,7 @@ def _bind(lib: ct.CDLL) -> None:
     lib.zodi_lonlat_to_vec.restype = ct.c_int32
     lib.zodi_lonlat_to_vec.argtypes = [_p, _p, _i64, _i32, _p, _p, _p]
     lib.zodi_spline_positions.restype = ct.c_int32
-    lib.zodi_spline_positions.argtypes = [_p, _i64, _p, _p, _i64, _p, _p]
+    lib.zodi_spline_positions.argtypes = [_p, _i64, _p, _i64, _p, _p]
     lib.zodi_host_device_pointer.restype = ct.c_int32
     lib.zodi_host_device_pointer.argtypes = [_p, ct.POINTER(ct.c_void_p)]
     lib.zodi_adjoint_workspace.restype = ct.c_int64

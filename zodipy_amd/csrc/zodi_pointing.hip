@@ -17,6 +17,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdint>
 
 #include "zodi_common.hpp"
 
@@ -40,60 +41,87 @@ __device__ __forceinline__ void store_rotated(const Rot3& R, bool identity, doub
   }
 }
 
-// (z, sin(theta), phi) of RING pixel p: rings 1 .. nside - 1 form the north cap
-// (4 i pixels each), rings nside .. 3 nside the belt (4 nside pixels each), the
-// rest the south cap. In the caps 1 - |z| = i^2 / (3 nside^2) gives sin(theta)
-// without the cancellation of 1 - z^2. The ring index from the square root is
-// corrected by one where rounding put it off.
-__device__ __forceinline__ void ring_pixel(int64_t ns, int64_t p, double& z, double& sth,
-                                           double& phi) {
-  const int64_t ncap = 2 * ns * (ns - 1);
-  const int64_t npix = 12 * ns * ns;
-  const double fact2 = 1.0 / (3.0 * (double)ns * (double)ns);
+// Per-launch constants of a RING-scheme map.
+struct RingGeometry {
+  int64_t ns, ncap, npix;
+  double fact2;    // 1 / (3 ns^2)
+  double inv_4ns;  // 1 / (4 ns), for the belt's ring estimate only
+};
+
+__device__ __forceinline__ RingGeometry ring_geometry(int64_t ns) {
+  RingGeometry g;
+  g.ns = ns;
+  g.ncap = 2 * ns * (ns - 1);
+  g.npix = 12 * ns * ns;
+  g.fact2 = 1.0 / (3.0 * (double)ns * (double)ns);
+  g.inv_4ns = 1.0 / (4.0 * (double)ns);
+  return g;
+}
+
+// (z, sin(theta), phi) of RING pixel p: rings 1 .. nside - 1 form the north
+// cap (4 i pixels each), rings nside .. 3 nside the belt (4 nside pixels each),
+// the rest the south cap. In the caps 1 - |z| = i^2 / (3 nside^2) gives
+// sin(theta) without the cancellation of 1 - z^2. The ring index from the
+// square root, and the belt's ring from a floating-point quotient (exact to
+// within one for every map of at most 12 * 2^58 pixels), are corrected by one
+// where rounding put them off; no 64-bit integer division is needed. z and phi
+// are formed with the same operations, in the same order, as the host's
+// zodipy_amd.healpix.pix2ang_ring.
+__device__ __forceinline__ void ring_pixel(const RingGeometry& g, int64_t p, double& z,
+                                           double& sth, double& phi) {
   const double half_pi = 1.5707963267948966;
-  if (p < ncap) {
+  if (p < g.ncap) {
     int64_t i = (1 + (int64_t)floor(sqrt(1.0 + 2.0 * (double)p))) / 2;
     if (2 * i * (i - 1) > p) --i;
     if (2 * (i + 1) * i <= p) ++i;
     const int64_t j = p + 1 - 2 * i * (i - 1);
-    const double t = (double)i * (double)i * fact2;
+    const double t = (double)i * (double)i * g.fact2;
     z = 1.0 - t;
     sth = sqrt(t * (2.0 - t));
     phi = ((double)j - 0.5) * half_pi / (double)i;
-  } else if (p >= npix - ncap) {
-    const int64_t q = npix - p;
+  } else if (p >= g.npix - g.ncap) {
+    const int64_t q = g.npix - p;
     int64_t i = (1 + (int64_t)floor(sqrt(2.0 * (double)q - 1.0))) / 2;
     if (2 * i * (i - 1) >= q) --i;
     if (2 * (i + 1) * i < q) ++i;
     const int64_t j = 4 * i + 1 - (q - 2 * i * (i - 1));
-    const double t = (double)i * (double)i * fact2;
+    const double t = (double)i * (double)i * g.fact2;
     z = -(1.0 - t);
     sth = sqrt(t * (2.0 - t));
     phi = ((double)j - 0.5) * half_pi / (double)i;
   } else {
-    const int64_t b = p - ncap;
-    const int64_t i = b / (4 * ns) + ns;
-    const int64_t j = b % (4 * ns) + 1;
-    const double fodd = ((i + ns) & 1) ? 1.0 : 0.5;
-    z = (double)(2 * ns - i) * (2.0 / 3.0) / (double)ns;
+    const int64_t b = p - g.ncap;
+    const int64_t w = 4 * g.ns;
+    int64_t r = (int64_t)((double)b * g.inv_4ns);  // ring within the belt
+    int64_t c = b - r * w;                          // pixel within the ring
+    if (c < 0) {
+      --r;
+      c += w;
+    } else if (c >= w) {
+      ++r;
+      c -= w;
+    }
+    const int64_t i = r + g.ns;
+    const double fodd = ((i + g.ns) & 1) ? 1.0 : 0.5;
+    z = (double)(2 * g.ns - i) * (2.0 / 3.0) / (double)g.ns;
     sth = sqrt((1.0 - z) * (1.0 + z));
-    phi = ((double)j - fodd) * half_pi / (double)ns;
+    phi = ((double)(c + 1) - fodd) * half_pi / (double)g.ns;
   }
 }
 
 __global__ __launch_bounds__(256) void pix2vec_kernel(int64_t nside, const int64_t* __restrict__ pix,
                                                       int64_t first, int64_t n, Rot3 R,
                                                       int identity, double* __restrict__ out) {
-  const int64_t npix = 12 * nside * nside;
+  const RingGeometry g = ring_geometry(nside);
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
     const int64_t p = pix != nullptr ? pix[i] : first + i;
     double x, y, z;
-    if (p < 0 || p >= npix) {
+    if (p < 0 || p >= g.npix) {
       x = y = z = __builtin_nan("");
     } else {
       double sth, phi, s, c;
-      ring_pixel(nside, p, z, sth, phi);
+      ring_pixel(g, p, z, sth, phi);
       sincos(phi, &s, &c);
       x = sth * c;
       y = sth * s;
@@ -102,6 +130,8 @@ __global__ __launch_bounds__(256) void pix2vec_kernel(int64_t nside, const int64
   }
 }
 
+// Degrees are scaled by the double nearest pi/180 before sincos, the rounding
+// of the reference's degree-to-radian conversion.
 __global__ __launch_bounds__(256) void lonlat_kernel(const double* __restrict__ lon,
                                                      const double* __restrict__ lat, int64_t n,
                                                      double scale, Rot3 R, int identity,
@@ -120,17 +150,32 @@ __global__ __launch_bounds__(256) void lonlat_kernel(const double* __restrict__ 
 // piecewise-polynomial form. The interval is scipy's (the last breakpoint at or
 // below t, clipped to the first and last intervals, which then extrapolate) and
 // the polynomial is summed in increasing powers of t - x_k as scipy does.
-// The interval is first guessed as if the grid were uniform (the hourly grids
-// of time-ordered data are), checked against its two breakpoints, and only
-// searched by bisection when the guess misses.
-__global__ __launch_bounds__(256) void spline_kernel(const double* __restrict__ x, int64_t m,
-                                                     const double* __restrict__ c,
+//
+// Each interval is one 128-byte record (kSplineRecord doubles): x_k, the
+// constant, linear, quadratic and cubic coefficients of x, y, z, then x_{k+1}
+// and two zeros. Times in random order then gather one cache line per ray
+// instead of six. The interval is first guessed as if the grid were uniform
+// (the hourly grids of time-ordered data are), checked against the record's
+// own two breakpoints, and searched by bisection only when the guess misses.
+constexpr int kSplineRecord = 16;
+
+__device__ __forceinline__ void load_record(const double* __restrict__ rec, int64_t k,
+                                            double (&v)[14]) {
+  const double2* r2 = reinterpret_cast<const double2*>(rec + k * kSplineRecord);
+#pragma unroll
+  for (int q = 0; q < 7; ++q) {
+    const double2 w = r2[q];
+    v[2 * q] = w.x;
+    v[2 * q + 1] = w.y;
+  }
+}
+
+__global__ __launch_bounds__(256) void spline_kernel(const double* __restrict__ rec, int64_t ni,
                                                      const double* __restrict__ t, int64_t n,
                                                      double* __restrict__ out) {
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  const int64_t ni = m - 1;  // intervals
-  const double x0 = x[0];
-  const double xl = x[m - 1];
+  const double x0 = rec[0];
+  const double xl = rec[(ni - 1) * kSplineRecord + 13];
   // Reached only with x0 <= t < xl, so the spacing is positive there; a guess
   // outside [0, ni - 1] (an infinite or NaN quotient) is clipped below.
   const double inv_step = (double)ni / (xl - x0);
@@ -144,32 +189,36 @@ __global__ __launch_bounds__(256) void spline_kernel(const double* __restrict__ 
     } else {
       const double g = (ti - x0) * inv_step;
       k = (g >= 0.0 && g < (double)(ni - 1)) ? (int64_t)g : ni - 1;
-      if (!(x[k] <= ti && ti < x[k + 1])) {
-        int64_t lo = 0, hi = m - 1;  // x[lo] <= ti < x[hi]
-        while (hi - lo > 1) {
-          const int64_t mid = (lo + hi) >> 1;
-          if (x[mid] <= ti) {
-            lo = mid;
-          } else {
-            hi = mid;
-          }
+    }
+    // The guessed record is loaded whole at once (one round trip to memory) and
+    // kept unless its breakpoints show the guess missed.
+    double v[14];
+    load_record(rec, k, v);
+    if (ti >= x0 && ti < xl && !(v[0] <= ti && ti < v[13])) {
+      int64_t lo = 0, hi = ni;  // breakpoints: x_lo <= ti < x_hi
+      while (hi - lo > 1) {
+        const int64_t mid = (lo + hi) >> 1;  // lo < mid < hi <= ni: a record's x_k
+        if (rec[mid * kSplineRecord] <= ti) {
+          lo = mid;
+        } else {
+          hi = mid;
         }
-        k = lo;
       }
+      load_record(rec, lo, v);
     }
     {
-      // c is [4][ni][3]; c[0] multiplies s^3. Separately rounded products and sums
-      // (no fused multiply-add), term by term in increasing powers, as scipy does.
+      // Separately rounded products and sums (no fused multiply-add), term by
+      // term in increasing powers, as scipy does.
 #pragma clang fp contract(off)
-      const double s = ti - x[k];
+      const double s = ti - v[0];
       const double s2 = s * s;
       const double s3 = s2 * s;
 #pragma unroll
       for (int d = 0; d < 3; ++d) {
-        double res = c[(3 * ni + k) * 3 + d];
-        res = res + c[(2 * ni + k) * 3 + d] * s;
-        res = res + c[(1 * ni + k) * 3 + d] * s2;
-        res = res + c[k * 3 + d] * s3;
+        double res = v[1 + d];
+        res = res + v[4 + d] * s;
+        res = res + v[7 + d] * s2;
+        res = res + v[10 + d] * s3;
         out[d * n + i] = res;
       }
     }
@@ -178,9 +227,13 @@ __global__ __launch_bounds__(256) void spline_kernel(const double* __restrict__ 
 
 namespace {
 
+// Up to 256 blocks of 256 threads per CU, each thread looping over a few
+// elements: on 50M-element inputs this beats both 16 blocks per CU (too few
+// loads in flight) and one element per thread (per-thread set-up no longer
+// amortised) by 5-15%.
 int64_t elementwise_grid(int64_t n) {
   const int64_t blocks = (n + 255) / 256;
-  return std::max<int64_t>(1, std::min<int64_t>(blocks, 16 * (int64_t)cu_count()));
+  return std::max<int64_t>(1, std::min<int64_t>(blocks, 256 * (int64_t)cu_count()));
 }
 
 void load_rotation(const double* rot, Rot3& R, int& identity) {
@@ -229,16 +282,18 @@ zodi_status zodi_lonlat_to_vec(const double* lon, const double* lat, int64_t n, 
   return check_hip(hipGetLastError(), "lonlat launch");
 }
 
-zodi_status zodi_spline_positions(const double* x, int64_t m, const double* c, const double* t,
+zodi_status zodi_spline_positions(const double* records, int64_t n_intervals, const double* t,
                                   int64_t n, double* out, void* stream) {
-  if (m < 2) return fail(ZODI_ERR_ARG, "a track needs at least 2 breakpoints, got %lld",
-                         (long long)m);
+  if (n_intervals < 1) return fail(ZODI_ERR_ARG, "a track needs at least 1 interval, got %lld",
+                                   (long long)n_intervals);
   if (n < 0) return fail(ZODI_ERR_ARG, "n < 0");
   if (n == 0) return ZODI_OK;
-  if (x == nullptr || c == nullptr || t == nullptr || out == nullptr)
+  if (records == nullptr || t == nullptr || out == nullptr)
     return fail(ZODI_ERR_ARG, "null pointer");
+  if (reinterpret_cast<uintptr_t>(records) % 16 != 0)
+    return fail(ZODI_ERR_ARG, "track records must be 16-byte aligned");
   hipLaunchKernelGGL(spline_kernel, dim3((unsigned)elementwise_grid(n)), dim3(256), 0,
-                     (hipStream_t)stream, x, m, c, t, n, out);
+                     (hipStream_t)stream, records, n_intervals, t, n, out);
   return check_hip(hipGetLastError(), "spline launch");
 }
 

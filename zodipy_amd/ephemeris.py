@@ -110,6 +110,22 @@ class BodyTrack:
         self.c = np.ascontiguousarray(self.spline.c, dtype=np.float64)  # (4, m - 1, 3)
         self._dev = {}
 
+    @property
+    def records(self) -> np.ndarray:
+        """The spline as the device kernel reads it: (m - 1, 16), one row per interval.
+
+        Row k holds x_k, the constant, linear, quadratic and cubic coefficients of
+        (x, y, z) in powers of t - x_k, x_{k+1} and two zeros: 128 bytes, one
+        cache line, so a ray's lookup touches a single line.
+        """
+        ni = self.x.size - 1
+        rec = np.zeros((ni, 16), dtype=np.float64)
+        rec[:, 0] = self.x[:-1]
+        for j in range(4):
+            rec[:, 1 + 3 * j:4 + 3 * j] = self.c[3 - j]
+        rec[:, 13] = self.x[1:]
+        return rec
+
     @classmethod
     def hourly(cls, body_position_fn, t0_mjd: float, t1_mjd: float) -> "BodyTrack":
         """Track on the reference's hourly grid over [t0, t1] from ``fn(mjd) -> (3, m)``."""
@@ -129,16 +145,14 @@ class BodyTrack:
         dev = _require_cuda(device)
         key = str(dev)
         if key not in self._dev:
-            self._dev[key] = (torch.as_tensor(self.x, device=dev),
-                              torch.as_tensor(self.c, device=dev))
-        x_t, c_t = self._dev[key]
+            self._dev[key] = torch.as_tensor(self.records, device=dev)
+        rec_t = self._dev[key]
         if isinstance(mjd, torch.Tensor):
             t = mjd.to(device=dev, dtype=torch.float64).reshape(-1).contiguous()
         else:
             t = torch.as_tensor(np.asarray(mjd, dtype=np.float64).ravel(), device=dev)
         out = torch.empty((3, t.numel()), dtype=torch.float64, device=dev)
-        st = _lib.load().zodi_spline_positions(x_t.data_ptr(), self.x.size, c_t.data_ptr(),
-                                               t.data_ptr(), t.numel(), out.data_ptr(),
-                                               _stream())
+        st = _lib.load().zodi_spline_positions(rec_t.data_ptr(), rec_t.shape[0], t.data_ptr(),
+                                               t.numel(), out.data_ptr(), _stream())
         _lib.check(st, "zodi_spline_positions")
         return out
