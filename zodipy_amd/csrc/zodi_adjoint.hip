@@ -19,7 +19,7 @@
 // butterfly (every lane obtains the same bits) and lane 0 adds the sums to the
 // wave's gradient row in LDS. At the end the block
 // adds its wave rows in wave order into one partial vector and a second kernel
-// adds the partials in block order: the result is reproducible bit for bit.
+// adds the partials in a fixed order: the result is reproducible bit for bit.
 //
 // The parameter vector follows zodipy/zodiacal_light_model.py:38-55 (component
 // init fields in declaration order) followed by the effective spectral
@@ -306,15 +306,25 @@ adjoint_layout(AdjArgs a) {
   }
 }
 
-// Sum of the block partials in block order.
+// One block per gradient entry: thread t sums block partials t, t + 256, ... in
+// order, then a fixed LDS tree adds the 256 sums. The order of every addition
+// is fixed by the launch shape alone, so results repeat bit for bit.
 __global__ __launch_bounds__(256) void adjoint_reduce(const double* __restrict__ partials,
                                                       int n_blocks, int np1,
                                                       double* __restrict__ grad) {
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= np1) return;
+  __shared__ double red[256];
+  const int p = blockIdx.x;
+  const int t = threadIdx.x;
   double s = 0.0;
-  for (int b = 0; b < n_blocks; ++b) s += partials[(int64_t)b * np1 + p];
-  grad[p] = s;
+  for (int b = t; b < n_blocks; b += 256) s += partials[(int64_t)b * np1 + p];
+  red[t] = s;
+  __syncthreads();
+#pragma unroll
+  for (int w = 128; w > 0; w >>= 1) {
+    if (t < w) red[t] += red[t + w];
+    __syncthreads();
+  }
+  if (t == 0) grad[p] = red[0];
 }
 
 namespace {
@@ -548,7 +558,7 @@ zodi_status zodi_adjoint(const zodi_model* model_host, const zodi_model* model_d
   hipLaunchKernelGGL(k, dim3((unsigned)plan.grid), dim3(kAdjBlock), lds, (hipStream_t)stream, a);
   st = check_hip(hipGetLastError(), "adjoint launch");
   if (st != ZODI_OK) return st;
-  hipLaunchKernelGGL(adjoint_reduce, dim3((np1 + 255) / 256), dim3(256), 0, (hipStream_t)stream,
+  hipLaunchKernelGGL(adjoint_reduce, dim3(np1), dim3(256), 0, (hipStream_t)stream,
                      (const double*)workspace, (int)plan.grid, np1, grad);
   return check_hip(hipGetLastError(), "adjoint reduce launch");
 }
