@@ -24,6 +24,8 @@
  *                         (the order of zodipy/zodiacal_light_model.py:38-55's
  *                         parameter dictionary, followed by the effective spectral
  *                         parameters), for parameter fitting
+ *   zodi_jacobian         no reference counterpart: the per-ray Jacobian of the
+ *                         same map, for Gauss-Newton and Fisher-matrix work
  */
 #ifndef ZODI_HIP_H
 #define ZODI_HIP_H
@@ -34,7 +36,7 @@
 extern "C" {
 #endif
 
-#define ZODI_ABI_VERSION 1
+#define ZODI_ABI_VERSION 2
 
 #define ZODI_MAX_COMPS 16
 #define ZODI_MAX_GROUPS 16
@@ -287,17 +289,22 @@ enum {
  * Rays are addressed as in zodi_forward. grad is [n_params + 1] in device memory;
  * entry n_params is chi2 in ZODI_ADJ_CHI2 mode and 0 otherwise. inv_var is
  * [n_rays] or NULL (then inv_var_scalar applies to every ray). In ZODI_ADJ_CHI2
- * mode the forward kernel first writes I [n_rays] to model_out, or to the end of
- * the workspace when model_out is NULL. workspace holds zodi_adjoint_workspace()
- * bytes of device memory for the same n_rays and launch. launch may be NULL;
- * otherwise block must be 0 or 256, lanes_per_ray 0 (chosen from n_rays), 1, 2 or
- * 4 (lanes sharing one ray's nodes), blocks_per_cu 0..64 (0 = 2), and variant the
+ * mode I [n_rays] is written to model_out, or to the workspace when model_out is
+ * NULL. workspace holds zodi_adjoint_workspace() bytes of device memory for the
+ * same n_rays, mode and launch. In ZODI_ADJ_CHI2 mode that size includes the
+ * per-ray Jacobian when it fits in 72 GiB and launch allows one lane per ray:
+ * one pass then writes the Jacobian and I (zodi_jacobian) and a second forms
+ * chi2 and J^T g from them; with less workspace, or a larger problem, the
+ * forward kernel writes I and the adjoint kernel recomputes the integrand with
+ * the weights g. The two paths agree to rounding. launch may be NULL; otherwise
+ * block must be 0 or 256, lanes_per_ray 0 (chosen from n_rays), 1, 2 or 4 (lanes
+ * sharing one ray's nodes), blocks_per_cu 0..64 (0 = 2), and variant the
  * components evaluated per pass over a group's nodes: 0 (default, 2), 1 or 2.
  * Supported layouts: the Kelsall registry models (cloud and three bands,
  * optionally followed by the ring and the feature); others return
  * ZODI_ERR_MODEL. The reduction order is fixed: equal inputs give equal bits.
  */
-int64_t zodi_adjoint_workspace(const zodi_model* model_host, int64_t n_rays,
+int64_t zodi_adjoint_workspace(const zodi_model* model_host, int64_t n_rays, int32_t mode,
                                const zodi_launch* launch);
 
 zodi_status zodi_adjoint(const zodi_model* model_host, const zodi_model* model_dev,
@@ -308,6 +315,21 @@ zodi_status zodi_adjoint(const zodi_model* model_host, const zodi_model* model_d
                          double inv_var_scalar, double* model_out, double* grad,
                          double* workspace, int64_t workspace_bytes, const zodi_launch* launch,
                          void* stream);
+
+/*
+ * zodi_jacobian — the per-ray Jacobian jac[p * ld_jac + r] = dI_r/dtheta_p
+ * ([n_params][ld_jac], ld_jac >= n_rays) and the summed emission I_r
+ * (model_out [n_rays]) in one pass, all in device memory. No reference
+ * counterpart (the reference has no derivatives); the parameter order is that
+ * of zodi_adjoint. For Gauss-Newton or Fisher-matrix work (J^T W J) and for the
+ * chi2 mode of zodi_adjoint. Rays, layouts and launch as in zodi_adjoint, with
+ * lanes_per_ray 0 or 1. I_r agrees with zodi_forward's to rounding.
+ */
+zodi_status zodi_jacobian(const zodi_model* model_host, const zodi_model* model_dev,
+                          const double* u, int64_t n_rays, const int64_t* seg_offsets,
+                          int32_t n_seg, const double* obs, const double* earth,
+                          int32_t obs_mode, const uint32_t* beyond, double* model_out,
+                          double* jac, int64_t ld_jac, const zodi_launch* launch, void* stream);
 
 #ifdef __cplusplus
 }

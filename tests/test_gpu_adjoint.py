@@ -128,15 +128,21 @@ def test_vjp_matches_central_fd(setup):
     sensitivity_check(g, fd, theta, 1e-6, 1e-6, "vjp vs central FD")
 
 
-def test_chi2_fused_matches_vjp(setup):
+@pytest.mark.parametrize("store_jacobian", [True, False], ids=["jacobian", "recompute"])
+def test_chi2_fused_matches_vjp(setup, store_jacobian):
     _, _, _, theta, v, prob = setup
+    prob.store_jacobian = store_jacobian
     rng = np.random.default_rng(4)
     I0 = prob.forward(theta)
     data = I0 * torch.as_tensor(1 + 0.01 * rng.normal(size=v.shape[1]), device="cuda")
     iv = torch.as_tensor(rng.uniform(0.5, 2.0, size=v.shape[1]), device="cuda")
     c2, g, model_vals = prob.chi2_and_grad(theta, data, iv, return_model=True)
-    assert torch.equal(model_vals, I0)  # written by the same forward kernel
-    res = I0 - data
+    prob.store_jacobian = True
+    if store_jacobian:  # I from the Jacobian pass: the forward's sums, rounded apart
+        torch.testing.assert_close(model_vals, I0, rtol=1e-13, atol=0)
+    else:
+        assert torch.equal(model_vals, I0)  # written by the same forward kernel
+    res = model_vals - data
     want_c2 = float((res * res * iv).sum())
     assert abs(float(c2) - want_c2) <= 1e-12 * want_c2
     g_vjp = prob.vjp(theta, 2 * res * iv)
@@ -147,6 +153,88 @@ def test_chi2_fused_matches_vjp(setup):
     c2t, gt = prob.chi2_and_grad(theta, data, torch.full_like(iv, 2.0))
     assert float(c2s) == float(c2t)
     assert torch.equal(gs, gt)
+
+
+def test_jacobian_matches_twin_forward_and_vjp(setup):
+    """Per-ray Jacobian: rows against the twin's autograd Jacobian, I against the
+    forward kernel, and J^T w against the adjoint's VJP."""
+    _, om, names, theta, v, prob = setup
+    I, J = prob.jacobian(theta)
+    assert J.shape == (prob.n_params, v.shape[1])
+    torch.testing.assert_close(I, prob.forward(theta), rtol=1e-13, atol=0)
+    w = torch.as_tensor(np.random.default_rng(14).normal(size=v.shape[1]), device="cuda")
+    sensitivity_check((J @ w).cpu().numpy(), prob.vjp(theta, w).cpu().numpy(), theta, 1e-11,
+                      1e-11, "J^T w vs vjp")
+    tw = KelsallTwin(om, names)
+    rays = [0, 5, 77, 200, 383, 700]
+    jt = torch.autograd.functional.jacobian(
+        lambda th: tw.emission(th, v[:, rays], OBS, EARTH).sum(0),
+        torch.tensor(theta, dtype=torch.float64))  # (len(rays), n_params)
+    Jc = J.cpu().numpy()
+    for k, r in enumerate(rays):
+        sensitivity_check(Jc[:, r], jt[k].numpy(), theta, 1e-8, 1e-8, f"J[:, {r}] vs twin")
+
+
+def test_jacobian_leading_dimension_and_chunks(setup):
+    """ld_jac > n_rays leaves the padding untouched; one or two components per pass agree."""
+    from zodipy_amd import _lib
+    from zodipy_amd._lib import Launch
+    from zodipy_amd.runtime import _stream
+
+    _, _, _, theta, v, prob = setup
+    n = v.shape[1]
+    I, J = prob.jacobian(theta)
+    host, dev, _ = prob._model(theta)
+    ld = n + 37
+    jp = torch.full((prob.n_params, ld), -7.0, dtype=torch.float64, device="cuda")
+    Ip = torch.empty(n, dtype=torch.float64, device="cuda")
+    st = prob.lib.zodi_jacobian(_lib.host_ptr(host), dev.data_ptr(), *prob._ptr_args(),
+                                Ip.data_ptr(), jp.data_ptr(), ld, None, _stream())
+    _lib.check(st, "zodi_jacobian")
+    assert torch.equal(jp[:, :n], J) and torch.equal(Ip, I)
+    assert torch.all(jp[:, n:] == -7.0)
+    prob.launch = Launch(0, 1, 2, 1)
+    try:
+        I1, J1 = prob.jacobian(theta)
+    finally:
+        prob.launch = None
+    torch.testing.assert_close(I1, I, rtol=1e-14, atol=0)
+    torch.testing.assert_close(J1, J, rtol=1e-12, atol=1e-14 * float(J.abs().max()))
+
+
+def test_chi2_paths_agree_on_segments_and_per_ray(gpu):
+    """Jacobian and recompute chi^2 paths on segmented (with an empty segment and an
+    observer beyond the ring) and per-ray (scattering) problems; both deterministic."""
+    x = 25 * u.micron
+    counts = [300, 0, 170, 257]
+    phases = [0.3, 1.0, 2.5, 4.0]
+    radii = [1.0, 1.0, 1.0, 1.25]
+    obs = np.stack([[r * np.cos(p), r * np.sin(p), 0.002 * k]
+                    for k, (r, p) in enumerate(zip(radii, phases))], axis=1)
+    earth = np.stack([[np.cos(p), np.sin(p), 0.0] for p in phases], axis=1)
+    seg = np.concatenate([[0], np.cumsum(counts)]).astype(np.int64)
+    v = random_dirs(seg[-1], 15)
+    cases = [(Model(x), dict(obspos=obs, earthpos=earth, seg_offsets=seg), v)]
+    n = 400
+    rng = np.random.default_rng(16)
+    ph = rng.uniform(0, 2 * np.pi, n)
+    o2 = np.stack([np.cos(ph), np.sin(ph), 0.001 * np.ones(n)])
+    e2 = np.stack([np.cos(ph), np.sin(ph), np.zeros(n)])
+    cases.append((Model(1.25 * u.micron), dict(obspos=o2, earthpos=e2, per_ray=True),
+                  random_dirs(n, 17)))
+    for model, kw, vv in cases:
+        prob = LineOfSightProblem(model, vv, **kw)
+        theta = prob.space.theta0
+        I0 = prob.forward(theta)
+        data = I0 * torch.as_tensor(1 + 0.02 * rng.normal(size=vv.shape[1]), device="cuda")
+        c_j, g_j = prob.chi2_and_grad(theta, data, 4.0)
+        c_j2, g_j2 = prob.chi2_and_grad(theta, data, 4.0)
+        assert torch.equal(c_j, c_j2) and torch.equal(g_j, g_j2)
+        prob.store_jacobian = False
+        c_r, g_r = prob.chi2_and_grad(theta, data, 4.0)
+        assert abs(float(c_j) - float(c_r)) <= 1e-11 * float(c_r)
+        sensitivity_check(g_j.cpu().numpy(), g_r.cpu().numpy(), theta, 1e-8, 1e-9,
+                          "jacobian vs recompute chi2 gradient")
 
 
 def test_adjoint_is_deterministic(setup):
@@ -245,6 +333,11 @@ def test_empty_ray_set(gpu):
     g = prob.vjp(prob.space.theta0, torch.zeros(0, dtype=torch.float64, device="cuda"))
     assert g.shape == (prob.n_params,)
     assert torch.all(g == 0)
+    empty = torch.zeros(0, dtype=torch.float64, device="cuda")
+    c2, g2 = prob.chi2_and_grad(prob.space.theta0, empty)
+    assert float(c2) == 0.0 and torch.all(g2 == 0)
+    I, J = prob.jacobian(prob.space.theta0)
+    assert I.shape == (0,) and J.shape == (prob.n_params, 0)
 
 
 def test_unsupported_layout_raises(gpu):

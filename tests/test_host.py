@@ -23,6 +23,7 @@ from zodipy_amd import units as u
 from zodipy_amd.component import ComponentLabel
 from zodipy_amd.fitting import ParameterSpace
 from zodipy_amd.packing import (
+    ABI_VERSION,
     CF_CENTERED,
     CF_P_IS_4,
     CF_SCATTER,
@@ -59,7 +60,7 @@ def test_library_exports_every_declared_function():
 
 def test_abi_version_and_model_size():
     lib = _lib.load()
-    assert lib.zodi_abi_version() == 1
+    assert lib.zodi_abi_version() == ABI_VERSION == 2
     assert lib.zodi_model_size() == MODEL_DTYPE.itemsize == 15848
 
 
@@ -143,11 +144,43 @@ def test_adjoint_workspace_size():
     blk = np.ascontiguousarray(_packed().block)
     # without a device the library assumes 256 CUs: grid = min(tiles / 4, 2 * 256) blocks
     n = 1_000_000
-    got = lib.zodi_adjoint_workspace(blk.ctypes.data, n, None)
-    assert got == (512 * 79 + n) * 8
-    assert lib.zodi_adjoint_workspace(blk.ctypes.data, 64, None) == (1 * 79 + 64) * 8
-    assert lib.zodi_adjoint_workspace(blk.ctypes.data, -1, None) == -1
-    assert lib.zodi_adjoint_workspace(blk.ctypes.data, n, ct.byref(_lib.Launch(0, 3, 0, 0))) == -1
+    ws = lib.zodi_adjoint_workspace
+    for mode in (_lib.ADJ_VJP_SUM, _lib.ADJ_VJP_COMPS):
+        assert ws(blk.ctypes.data, n, mode, None) == (512 * 79 + n) * 8
+    assert ws(blk.ctypes.data, 64, 0, None) == (1 * 79 + 64) * 8
+    # chi^2 stores the Jacobian [78][n], the J^T g partials of 8192-ray chunks, then I
+    chunks = -(-n // 8192)
+    assert ws(blk.ctypes.data, n, _lib.ADJ_CHI2, None) == (78 * n + chunks * 79 + n) * 8
+    # ... unless the launch asks for lanes sharing a ray, or J would pass 72 GiB
+    two = ct.byref(_lib.Launch(0, 2, 0, 0))
+    assert ws(blk.ctypes.data, n, _lib.ADJ_CHI2, two) == ws(blk.ctypes.data, n, 0, two)
+    big = 130_000_000  # 78 x 1.3e8 x 8 B = 81 GB
+    assert ws(blk.ctypes.data, big, _lib.ADJ_CHI2, None) == (512 * 79 + big) * 8
+    assert ws(blk.ctypes.data, -1, 0, None) == -1
+    assert ws(blk.ctypes.data, n, 0, ct.byref(_lib.Launch(0, 3, 0, 0))) == -1
+    assert ws(blk.ctypes.data, n, 9, None) == -1
+
+
+@cpu_only
+def test_jacobian_host_validation_errors():
+    lib = _lib.load()
+    blk = np.ascontiguousarray(_packed().block)
+    d = ct.c_void_p(0x1000)  # never dereferenced: every case fails host validation
+
+    def call(block, n=4, out=d, jac=d, ld=4, launch=None):
+        return lib.zodi_jacobian(block.ctypes.data if block is not None else None, d, d, n, d, 1,
+                                 d, d, 0, d, out, jac, ld, launch, None)
+
+    assert call(None) == 1
+    assert call(blk, ld=3) == 1
+    assert b"ld_jac" in lib.zodi_last_error()
+    assert call(blk, jac=None) == 1
+    assert call(blk, launch=ct.byref(_lib.Launch(0, 2, 0, 0))) == 1
+    assert b"one lane per ray" in lib.zodi_last_error()
+    rrm = np.ascontiguousarray(_packed("rrm-experimental", 1.25 * u.micron,
+                                       extrapolate=True).block)
+    assert call(rrm) == 2
+    assert call(blk, n=0, out=None, jac=None, ld=0) == 0  # nothing to do
 
 
 # --------------------------------------------------------------------- packer

@@ -33,6 +33,7 @@ EXPORTED_SYMBOLS = (
     "zodi_host_device_pointer",
     "zodi_adjoint_workspace",
     "zodi_adjoint",
+    "zodi_jacobian",
 )
 
 
@@ -77,10 +78,13 @@ def _bind(lib: ct.CDLL) -> None:
     lib.zodi_host_device_pointer.restype = ct.c_int32
     lib.zodi_host_device_pointer.argtypes = [_p, ct.POINTER(ct.c_void_p)]
     lib.zodi_adjoint_workspace.restype = ct.c_int64
-    lib.zodi_adjoint_workspace.argtypes = [_p, _i64, ct.POINTER(Launch)]
+    lib.zodi_adjoint_workspace.argtypes = [_p, _i64, _i32, ct.POINTER(Launch)]
     lib.zodi_adjoint.restype = ct.c_int32
     lib.zodi_adjoint.argtypes = [_p, _p, _p, _i64, _p, _i32, _p, _p, _i32, _p, _i32, _p, _p,
                                  _p, ct.c_double, _p, _p, _p, _i64, ct.POINTER(Launch), _p]
+    lib.zodi_jacobian.restype = ct.c_int32
+    lib.zodi_jacobian.argtypes = [_p, _p, _p, _i64, _p, _i32, _p, _p, _i32, _p, _p, _p, _i64,
+                                  ct.POINTER(Launch), _p]
 
 
 def load() -> ct.CDLL:

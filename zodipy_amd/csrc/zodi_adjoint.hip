@@ -49,6 +49,9 @@ struct AdjArgs {
   double inv_var_scalar;
   const double* model_vals;  // CHI2: model values I [n_rays] (forward kernel output)
   double* partials;          // [gridDim.x][n_params + 1]
+  double* jac;               // Jacobian pass: [n_params][ld_jac]
+  int64_t ld_jac;
+  double* model_vals_out;    // Jacobian pass: I [n_rays]
   int32_t n_params;
   int32_t mode;              // ZODI_ADJ_*
 };
@@ -81,11 +84,20 @@ struct CompAcc {
 
 // Components J (consecutive, from S) of group g, whose first component is `first`.
 // Lane `sub` of the SPLIT lanes sharing the ray takes nodes sub, sub + SPLIT, ...
-template <int INTEGRAND, int SPLIT, int S, int... Ks, int... J>
+// `sh` holds the shared-parameter sums of the ray (Kelsall: T0, delta, F, C1..C3;
+// RRM: calibration) across all groups.
+//
+// JAC (one lane per ray): the weights are 1, and each lane stores its ray's sums
+// as a column of the Jacobian, dI_r/dtheta_p -> jac[p][r], instead of reducing
+// them over the wave; `total` receives I_r, each component's node sum finished
+// as the forward kernel finishes it and added in component order.
+template <int INTEGRAND, int SPLIT, bool JAC, int S, int... Ks, int... J>
 __device__ __forceinline__ void adjoint_group(Grp<Ks...>, std::integer_sequence<int, J...>,
                                               const Ctx& cx, const AdjArgs& a, const Ray& y,
                                               int g, int first, double gscale, double* row,
-                                              int lane, int sub) {
+                                              int lane, int sub, double (&sh)[6],
+                                              double& total) {
+  static_assert(!JAC || SPLIT == 1, "the Jacobian pass uses one lane per ray");
   constexpr int C = sizeof...(Ks);
   constexpr int N = sizeof...(J);
   constexpr int kinds[C] = {Ks...};
@@ -97,7 +109,9 @@ __device__ __forceinline__ void adjoint_group(Grp<Ks...>, std::integer_sequence<
   const double inv_T0 = 1.0 / T0;
 
   double W[N];  // h times the weight of each component
-  if (a.mode == ZODI_ADJ_VJP_COMPS) {
+  if constexpr (JAC) {
+    ((W[J - S] = y.active ? h : 0.0), ...);
+  } else if (a.mode == ZODI_ADJ_VJP_COMPS) {
     ((W[J - S] = y.active ? h * a.weights[(int64_t)(first + J) * a.f.n_rays + y.r] : 0.0), ...);
   } else {
     ((W[J - S] = h * gscale), ...);
@@ -122,7 +136,8 @@ __device__ __forceinline__ void adjoint_group(Grp<Ks...>, std::integer_sequence<
   const double phi_const = M->shared[ZODI_S_PHASE_NORM] * (M->shared[ZODI_S_C1] + 1.0);
 
   std::tuple<CompAcc<kind_nparams<kinds[J]>()>...> acc{};
-  double sh[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};  // Kelsall: T0, delta, F, C1..C3; RRM: cal
+  double Iv[N];  // JAC: node sums of each component's integrand, as the forward forms them
+  ((Iv[J - S] = 0.0), ...);
 
   const int K = y.active ? cx.K : 0;
   for (int k = sub; k < K; k += SPLIT) {
@@ -166,6 +181,9 @@ __device__ __forceinline__ void adjoint_group(Grp<Ks...>, std::integer_sequence<
             const double scat_unit = (solar * inv_R2) * pg.phi;  // F Phi / R^2
             double e = oma * (eps * nd.B);
             if (flags & ZODI_CF_SCATTER) e += alb * scat_unit;
+            if constexpr (JAC) {
+              Iv[J - S] += (flags & ZODI_CF_SCATTER) ? wk * (e * dg.n) : (wk * nd.B) * dg.n;
+            }
             const double we = wn * e;
 #pragma unroll
             for (int q = 0; q < 5; ++q) A[q] += we * geo[q];
@@ -185,6 +203,7 @@ __device__ __forceinline__ void adjoint_group(Grp<Ks...>, std::integer_sequence<
             sh[5] += wasf * pg.d3;
           } else {
             // f = calibration B(T_c) n   (zodipy/brightness.py:59-83)
+            if constexpr (JAC) Iv[J - S] += (wk * nd.B) * dg.n;
             const double wc = wn * cx.calibration;
             const double wcb = wc * nd.B;
 #pragma unroll
@@ -200,29 +219,44 @@ __device__ __forceinline__ void adjoint_group(Grp<Ks...>, std::integer_sequence<
         ...);
   }
 
-  // Wave sums into the wave's gradient row (lane 0 writes).
-  (
-      [&] {
-        constexpr int NP = kind_nparams<kinds[J]>();
-        auto& A = std::get<J - S>(acc).v;
-#pragma unroll
-        for (int q = 0; q < 5 + NP + 2; ++q) A[q] = wave_sum(A[q]);
-        if (lane == 0) {
+  if constexpr (JAC) {
+    // The ray's Jacobian entries of these components (coalesced over the wave).
+    (
+        [&] {
+          constexpr int NP = kind_nparams<kinds[J]>();
+          const auto& A = std::get<J - S>(acc).v;
           const int c = first + J;
-          const int off = M->grad_comp[c];
+          total = __dadd_rn(total, finish<INTEGRAND>(M->comp[c], M->comp_flags[c], Iv[J - S], h,
+                                                     cx.calibration));
+          if (y.active) {
+            double* col = a.jac + y.r;
+            const int64_t ld = a.ld_jac;
+            const int off = M->grad_comp[c];
 #pragma unroll
-          for (int q = 0; q < 5 + NP; ++q) row[off + q] += A[q];
-          row[M->grad_spec_a[c]] += A[5 + NP];
-          row[M->grad_spec_b[c]] += A[5 + NP + 1];
-        }
-      }(),
-      ...);
-  constexpr int NSH = INTEGRAND == ZODI_INTEGRAND_KELSALL ? 6 : 1;
+            for (int q = 0; q < 5 + NP; ++q) col[(off + q) * ld] = A[q];
+            col[M->grad_spec_a[c] * ld] = A[5 + NP];
+            col[M->grad_spec_b[c] * ld] = A[5 + NP + 1];
+          }
+        }(),
+        ...);
+  } else {
+    // Wave sums into the wave's gradient row (lane 0 writes).
+    (
+        [&] {
+          constexpr int NP = kind_nparams<kinds[J]>();
+          auto& A = std::get<J - S>(acc).v;
 #pragma unroll
-  for (int q = 0; q < NSH; ++q) sh[q] = wave_sum(sh[q]);
-  if (lane == 0) {
+          for (int q = 0; q < 5 + NP + 2; ++q) A[q] = wave_sum(A[q]);
+          if (lane == 0) {
+            const int c = first + J;
+            const int off = M->grad_comp[c];
 #pragma unroll
-    for (int q = 0; q < NSH; ++q) row[M->grad_shared[q]] += sh[q];
+            for (int q = 0; q < 5 + NP; ++q) row[off + q] += A[q];
+            row[M->grad_spec_a[c]] += A[5 + NP];
+            row[M->grad_spec_b[c]] += A[5 + NP + 1];
+          }
+        }(),
+        ...);
   }
 }
 
@@ -232,37 +266,62 @@ constexpr std::integer_sequence<int, (S + I)...> shift_seq(std::integer_sequence
 }
 
 // Group g in chunks of at most CHUNK components, from component S of the group.
-template <int INTEGRAND, int SPLIT, int CHUNK, int S, int... Ks>
+template <int INTEGRAND, int SPLIT, int CHUNK, bool JAC, int S, int... Ks>
 __device__ __forceinline__ void adjoint_chunks(Grp<Ks...> grp, const Ctx& cx, const AdjArgs& a,
                                                const Ray& y, int g, int first, double gscale,
-                                               double* row, int lane, int sub) {
+                                               double* row, int lane, int sub, double (&sh)[6],
+                                               double& total) {
   constexpr int C = sizeof...(Ks);
   if constexpr (S < C) {
     constexpr int N = (C - S) < CHUNK ? (C - S) : CHUNK;
-    adjoint_group<INTEGRAND, SPLIT, S>(grp, shift_seq<S>(std::make_integer_sequence<int, N>{}),
-                                       cx, a, y, g, first, gscale, row, lane, sub);
-    adjoint_chunks<INTEGRAND, SPLIT, CHUNK, S + N>(grp, cx, a, y, g, first, gscale, row, lane,
-                                                   sub);
+    adjoint_group<INTEGRAND, SPLIT, JAC, S>(
+        grp, shift_seq<S>(std::make_integer_sequence<int, N>{}), cx, a, y, g, first, gscale, row,
+        lane, sub, sh, total);
+    adjoint_chunks<INTEGRAND, SPLIT, CHUNK, JAC, S + N>(grp, cx, a, y, g, first, gscale, row,
+                                                        lane, sub, sh, total);
   }
 }
 
-template <int SPLIT, int CHUNK, int INTEGRAND, class... Gs>
+// Every group of the layout for one ray; then the shared-parameter sums go to the
+// wave's gradient row (wave sum, lane 0 writes) or, in the Jacobian pass, to the
+// ray's Jacobian column together with I_r.
+template <int SPLIT, int CHUNK, bool JAC, int INTEGRAND, class... Gs>
 __device__ __forceinline__ void run_adjoint(Layout<INTEGRAND, Gs...>, const Ctx& cx,
                                             const AdjArgs& a, const Ray& y, double gscale,
                                             double* row, int lane, int sub) {
+  double sh[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  double total = 0.0;
   int g = 0;
   int first = 0;
-  ((adjoint_chunks<INTEGRAND, SPLIT, CHUNK, 0>(Gs{}, cx, a, y, g, first, gscale, row, lane, sub),
+  ((adjoint_chunks<INTEGRAND, SPLIT, CHUNK, JAC, 0>(Gs{}, cx, a, y, g, first, gscale, row, lane,
+                                                    sub, sh, total),
     ++g, first += Gs::size),
    ...);
+  const ModelPtr M = cx.M;
+  constexpr int NSH = INTEGRAND == ZODI_INTEGRAND_KELSALL ? 6 : 1;
+  if constexpr (JAC) {
+    if (y.active) {
+      a.model_vals_out[y.r] = total;
+#pragma unroll
+      for (int q = 0; q < NSH; ++q) a.jac[M->grad_shared[q] * a.ld_jac + y.r] = sh[q];
+    }
+  } else {
+#pragma unroll
+    for (int q = 0; q < NSH; ++q) sh[q] = wave_sum(sh[q]);
+    if (lane == 0) {
+#pragma unroll
+      for (int q = 0; q < NSH; ++q) row[M->grad_shared[q]] += sh[q];
+    }
+  }
 }
 
-// One wave per tile of 64 / SPLIT rays; SPLIT lanes share a ray's nodes.
-template <class LAYOUT, int CHUNK, int SPLIT>
+// One wave per tile of 64 / SPLIT rays; SPLIT lanes share a ray's nodes. JAC: the
+// Jacobian pass (one lane per ray, no weights, no reduction).
+template <class LAYOUT, int CHUNK, int SPLIT, bool JAC>
 __global__ __launch_bounds__(kAdjBlock) __attribute__((amdgpu_waves_per_eu(2, 2))) void
 adjoint_layout(AdjArgs a) {
   constexpr int RAYS = kWave / SPLIT;
-  extern __shared__ double sRows[];  // kAdjWaves rows of n_params + 1
+  extern __shared__ double sRows[];  // kAdjWaves rows of n_params + 1 (not used by JAC)
   __shared__ double sTab[4 * ZODI_N_TEMPS];
   const ModelPtr M = as_const(a.f.m);
   const int tid = threadIdx.x;
@@ -270,16 +329,22 @@ adjoint_layout(AdjArgs a) {
   const int lane = tid % kWave;
   const int sub = lane % SPLIT;
   const int np1 = a.n_params + 1;
-  for (int i = tid; i < np1 * kAdjWaves; i += kAdjBlock) sRows[i] = 0.0;
+  if constexpr (!JAC) {
+    for (int i = tid; i < np1 * kAdjWaves; i += kAdjBlock) sRows[i] = 0.0;
+  }
   stage_math_tables();
   load_table(M, sTab);  // ends with a barrier
   const Ctx cx = make_ctx(M, sTab, 0);
-  double* row = sRows + wave * np1;
+  double* row = JAC ? nullptr : sRows + wave * np1;
 
   const int64_t stride = (int64_t)gridDim.x * kAdjWaves;
   for (int64_t tile = (int64_t)blockIdx.x * kAdjWaves + wave; tile < a.f.n_tiles;
        tile += stride) {
     const Ray y = load_ray<RAYS>(a.f, tile, lane / SPLIT);
+    if constexpr (JAC) {
+      run_adjoint<SPLIT, CHUNK, true>(LAYOUT{}, cx, a, y, 1.0, row, lane, sub);
+      continue;
+    }
     double gscale = 0.0;
     if (a.mode == ZODI_ADJ_CHI2) {
       double chi = 0.0;
@@ -295,8 +360,9 @@ adjoint_layout(AdjArgs a) {
     } else if (a.mode == ZODI_ADJ_VJP_SUM) {
       gscale = y.active ? a.weights[y.r] : 0.0;
     }
-    run_adjoint<SPLIT, CHUNK>(LAYOUT{}, cx, a, y, gscale, row, lane, sub);
+    run_adjoint<SPLIT, CHUNK, false>(LAYOUT{}, cx, a, y, gscale, row, lane, sub);
   }
+  if constexpr (JAC) return;
   __syncthreads();
   for (int p = tid; p < np1; p += kAdjBlock) {
     double s = sRows[p];
@@ -327,42 +393,105 @@ __global__ __launch_bounds__(256) void adjoint_reduce(const double* __restrict__
   if (t == 0) grad[p] = red[0];
 }
 
+// chi^2 and its gradient from a stored Jacobian: per ray g_r = 2 (I_r - d_r) w_r
+// with w_r = 1/sigma_r^2, grad_p = sum_r g_r J[p][r], chi^2 = sum_r (I_r - d_r)^2 w_r.
+// Block (x, y) takes rays [x R, x R + R) (R = kJacRays) and parameters
+// [y P, y P + P): thread t sums rays t, t + 256, ... of the range in order, each
+// wave sums its lanes with the xor butterfly, and the waves' sums are added in
+// wave order into partials[x][p] (entry n_params: chi^2, from the y = 0 blocks).
+// adjoint_reduce then adds the partials of all x in a fixed order.
+constexpr int kJacParams = 32;
+constexpr int64_t kJacRays = 8192;
+
+__global__ __launch_bounds__(256) void jacobian_chi2(const double* __restrict__ jac, int64_t ld,
+                                                     int n_params, int64_t n_rays,
+                                                     const double* __restrict__ vals,
+                                                     const double* __restrict__ target,
+                                                     const double* __restrict__ inv_var,
+                                                     double inv_var_scalar,
+                                                     double* __restrict__ partials) {
+  __shared__ double red[kAdjWaves][kJacParams + 1];
+  const int t = threadIdx.x;
+  const int wave = t / kWave, lane = t % kWave;
+  const int p0 = blockIdx.y * kJacParams;
+  const int np = min(kJacParams, n_params - p0);
+  const int64_t r0 = (int64_t)blockIdx.x * kJacRays;
+  const int64_t r1 = min(n_rays, r0 + kJacRays);
+  double acc[kJacParams];
+#pragma unroll
+  for (int q = 0; q < kJacParams; ++q) acc[q] = 0.0;
+  double chi = 0.0;
+  for (int64_t r = r0 + t; r < r1; r += 256) {
+    const double res = vals[r] - target[r];
+    const double iv = inv_var != nullptr ? inv_var[r] : inv_var_scalar;
+    chi += res * res * iv;
+    const double g = 2.0 * res * iv;
+    const double* col = jac + (int64_t)p0 * ld + r;
+#pragma unroll
+    for (int q = 0; q < kJacParams; ++q) {
+      if (q < np) acc[q] += g * col[q * ld];
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < kJacParams; ++q) {
+    const double v = wave_sum(acc[q]);
+    if (lane == 0) red[wave][q] = v;
+  }
+  chi = wave_sum(chi);
+  if (lane == 0) red[wave][kJacParams] = chi;
+  __syncthreads();
+  double* out = partials + (int64_t)blockIdx.x * (n_params + 1);
+  if (t < np) {
+    double s = red[0][t];
+#pragma unroll
+    for (int w = 1; w < kAdjWaves; ++w) s += red[w][t];
+    out[p0 + t] = s;
+  }
+  if (blockIdx.y == 0 && t == 0) {
+    double s = red[0][kJacParams];
+#pragma unroll
+    for (int w = 1; w < kAdjWaves; ++w) s += red[w][kJacParams];
+    out[n_params] = s;
+  }
+}
+
 namespace {
 
 using AdjKernel = void (*)(AdjArgs);
 
 template <class LAYOUT, int CHUNK>
-AdjKernel pick_adj_split(int split) {
+AdjKernel pick_adj_split(int split, bool jac) {
+  if (jac) return split == 1 ? adjoint_layout<LAYOUT, CHUNK, 1, true> : nullptr;
   switch (split) {
     case 1:
-      return adjoint_layout<LAYOUT, CHUNK, 1>;
+      return adjoint_layout<LAYOUT, CHUNK, 1, false>;
     case 2:
-      return adjoint_layout<LAYOUT, CHUNK, 2>;
+      return adjoint_layout<LAYOUT, CHUNK, 2, false>;
     case 4:
-      return adjoint_layout<LAYOUT, CHUNK, 4>;
+      return adjoint_layout<LAYOUT, CHUNK, 4, false>;
     default:
       return nullptr;
   }
 }
 
 template <class LAYOUT>
-AdjKernel pick_adj_chunk(int chunk, int split) {
+AdjKernel pick_adj_chunk(int chunk, int split, bool jac) {
   switch (chunk) {
     case 1:
-      return pick_adj_split<LAYOUT, 1>(split);
+      return pick_adj_split<LAYOUT, 1>(split, jac);
     case 2:
-      return pick_adj_split<LAYOUT, 2>(split);
+      return pick_adj_split<LAYOUT, 2>(split, jac);
     default:
       return nullptr;
   }
 }
 
-AdjKernel pick_adjoint(const zodi_model* m, int chunk, int split) {
+AdjKernel pick_adjoint(const zodi_model* m, int chunk, int split, bool jac) {
   switch (match_layout(m)) {
     case kLayoutDirbe:
-      return pick_adj_chunk<LayoutDirbe>(chunk, split);
+      return pick_adj_chunk<LayoutDirbe>(chunk, split, jac);
     case kLayoutKelsall4:
-      return pick_adj_chunk<LayoutKelsall4>(chunk, split);
+      return pick_adj_chunk<LayoutKelsall4>(chunk, split, jac);
     default:
       return nullptr;
   }
@@ -428,8 +557,9 @@ struct AdjPlan {
 // Launch plan. With lanes_per_ray = 0 one lane owns a ray unless the rays do not
 // fill every resident wave once; then the smallest split (2 or 4 lanes per ray)
 // that does. Each extra lane per ray repeats the per-ray set-up and the wave
-// reductions, which costs more than a partly idle last round.
-zodi_status adjoint_plan(const zodi_launch* launch, int64_t n_rays, AdjPlan& p) {
+// reductions, which costs more than a partly idle last round. The Jacobian pass
+// (jac) always uses one lane per ray.
+zodi_status adjoint_plan(const zodi_launch* launch, int64_t n_rays, bool jac, AdjPlan& p) {
   p.chunk = kDefaultChunk;
   p.split = 0;
   p.per_cu = kDefaultBlocksPerCu;
@@ -449,6 +579,12 @@ zodi_status adjoint_plan(const zodi_launch* launch, int64_t n_rays, AdjPlan& p) 
     p.split = launch->lanes_per_ray;
     if (launch->blocks_per_cu > 0) p.per_cu = launch->blocks_per_cu;
   }
+  if (jac) {
+    if (p.split > 1)
+      return fail(ZODI_ERR_ARG, "the Jacobian pass uses one lane per ray, got lanes_per_ray %d",
+                  p.split);
+    p.split = 1;
+  }
   const int64_t waves = (int64_t)p.per_cu * cu_count() * kAdjWaves;
   if (p.split == 0) {
     p.split = 4;
@@ -465,73 +601,32 @@ zodi_status adjoint_plan(const zodi_launch* launch, int64_t n_rays, AdjPlan& p) 
   return ZODI_OK;
 }
 
-}  // namespace
-}  // namespace zodi
+// Largest Jacobian the chi^2 mode stores in its workspace (a quarter of one
+// MI355X's 288 GB of HBM); larger problems recompute the integrand instead.
+constexpr int64_t kJacobianMaxBytes = int64_t(72) << 30;
 
-using namespace zodi;
-
-extern "C" {
-
-int64_t zodi_adjoint_workspace(const zodi_model* model_host, int64_t n_rays,
-                               const zodi_launch* launch) {
-  if (model_host == nullptr || validate_model(model_host) != ZODI_OK || n_rays < 0) return -1;
-  AdjPlan p;
-  if (adjoint_plan(launch, n_rays, p) != ZODI_OK) return -1;
-  return (p.grid * (int64_t)(model_host->n_params + 1) + n_rays) * (int64_t)sizeof(double);
+// Workspace of the recompute path: block partials, then I.
+int64_t recompute_bytes(const AdjPlan& p, int np1, int64_t n_rays) {
+  return (p.grid * np1 + n_rays) * (int64_t)sizeof(double);
 }
 
-zodi_status zodi_adjoint(const zodi_model* model_host, const zodi_model* model_dev,
-                         const double* u, int64_t n_rays, const int64_t* seg_offsets,
-                         int32_t n_seg, const double* obs, const double* earth, int32_t obs_mode,
-                         const uint32_t* beyond, int32_t mode, const double* weights,
-                         const double* target, const double* inv_var, double inv_var_scalar,
-                         double* model_out, double* grad, double* workspace,
-                         int64_t workspace_bytes, const zodi_launch* launch, void* stream) {
-  if (model_host == nullptr) return fail(ZODI_ERR_ARG, "null model");
-  zodi_status st = validate_model(model_host);
-  if (st != ZODI_OK) return st;
-  if (n_rays < 0) return fail(ZODI_ERR_ARG, "n_rays < 0");
-  if (model_dev == nullptr || grad == nullptr || workspace == nullptr)
-    return fail(ZODI_ERR_ARG, "null pointer (model_dev, grad or workspace)");
-  if (n_rays > 0 && (u == nullptr || obs == nullptr || earth == nullptr || beyond == nullptr))
-    return fail(ZODI_ERR_ARG, "null pointer (u, obs, earth or beyond)");
-  if (obs_mode == ZODI_OBS_PER_SEGMENT) {
-    if (n_rays > 0 && (seg_offsets == nullptr || n_seg < 1))
-      return fail(ZODI_ERR_ARG, "per-segment mode needs seg_offsets and n_seg >= 1");
-  } else if (obs_mode != ZODI_OBS_PER_RAY) {
-    return fail(ZODI_ERR_ARG, "unknown obs_mode %d", obs_mode);
-  }
-  if (mode == ZODI_ADJ_CHI2) {
-    if (n_rays > 0 && target == nullptr) return fail(ZODI_ERR_ARG, "chi2 mode needs target");
-  } else if (mode == ZODI_ADJ_VJP_SUM || mode == ZODI_ADJ_VJP_COMPS) {
-    if (n_rays > 0 && weights == nullptr) return fail(ZODI_ERR_ARG, "vjp mode needs weights");
-  } else {
-    return fail(ZODI_ERR_ARG, "unknown adjoint mode %d", mode);
-  }
-  if (model_host->n_params < 1 || model_host->n_params > ZODI_MAX_PARAMS)
-    return fail(ZODI_ERR_MODEL, "n_params %d outside [1, %d]", model_host->n_params,
-                ZODI_MAX_PARAMS);
-  AdjPlan plan;
-  st = adjoint_plan(launch, n_rays, plan);
-  if (st != ZODI_OK) return st;
-  AdjKernel k = pick_adjoint(model_host, plan.chunk, plan.split);
-  if (k == nullptr)
-    return fail(ZODI_ERR_MODEL,
-                "no adjoint for this component layout (supported: cloud, 3 bands[, ring, "
-                "feature] with the Kelsall integrand)");
-  st = validate_grad_layout(model_host);
-  if (st != ZODI_OK) return st;
-  const int np1 = model_host->n_params + 1;
-  const int64_t need = (plan.grid * np1 + n_rays) * (int64_t)sizeof(double);
-  if (workspace_bytes < need)
-    return fail(ZODI_ERR_ARG, "workspace of %lld bytes needed, %lld given", (long long)need,
-                (long long)workspace_bytes);
-  double* vals = model_out != nullptr ? model_out : workspace + plan.grid * np1;
-  if (mode == ZODI_ADJ_CHI2 && n_rays > 0) {
-    st = zodi_forward(model_host, model_dev, u, n_rays, seg_offsets, n_seg, obs, earth, obs_mode,
-                      beyond, vals, ZODI_OUT_SUM, nullptr, stream);
-    if (st != ZODI_OK) return st;
-  }
+// Workspace of the chi^2 mode's Jacobian path: J [n_params][n_rays], the
+// partials of jacobian_chi2, then I.
+int64_t jacobian_bytes(int np, int64_t n_rays) {
+  const int64_t chunks = (n_rays + kJacRays - 1) / kJacRays;
+  return ((int64_t)np * n_rays + chunks * (np + 1) + n_rays) * (int64_t)sizeof(double);
+}
+
+// The chi^2 mode can take the Jacobian path: one lane per ray allowed, J small enough.
+bool jacobian_eligible(const zodi_launch* launch, int np, int64_t n_rays) {
+  if (launch != nullptr && launch->lanes_per_ray > 1) return false;
+  return (int64_t)np * n_rays * (int64_t)sizeof(double) <= kJacobianMaxBytes;
+}
+
+AdjArgs make_args(const zodi_model* model_dev, const double* u, int64_t n_rays,
+                  const int64_t* seg_offsets, int32_t n_seg, const double* obs,
+                  const double* earth, int32_t obs_mode, const uint32_t* beyond,
+                  const AdjPlan& plan, int n_params) {
   AdjArgs a;
   memset(&a, 0, sizeof(a));
   a.f.m = model_dev;
@@ -546,13 +641,177 @@ zodi_status zodi_adjoint(const zodi_model* model_host, const zodi_model* model_d
   a.f.beyond = beyond;
   a.f.out_mode = ZODI_OUT_SUM;
   a.f.n_tiles = plan.n_tiles;
+  a.n_params = n_params;
+  return a;
+}
+
+// Checks shared by zodi_adjoint and zodi_jacobian.
+zodi_status check_rays(const zodi_model* model_host, const zodi_model* model_dev,
+                       const double* u, int64_t n_rays, const int64_t* seg_offsets,
+                       int32_t n_seg, const double* obs, const double* earth, int32_t obs_mode,
+                       const uint32_t* beyond) {
+  if (model_host == nullptr) return fail(ZODI_ERR_ARG, "null model");
+  zodi_status st = validate_model(model_host);
+  if (st != ZODI_OK) return st;
+  if (n_rays < 0) return fail(ZODI_ERR_ARG, "n_rays < 0");
+  if (model_dev == nullptr) return fail(ZODI_ERR_ARG, "null pointer (model_dev)");
+  if (n_rays > 0 && (u == nullptr || obs == nullptr || earth == nullptr || beyond == nullptr))
+    return fail(ZODI_ERR_ARG, "null pointer (u, obs, earth or beyond)");
+  if (obs_mode == ZODI_OBS_PER_SEGMENT) {
+    if (n_rays > 0 && (seg_offsets == nullptr || n_seg < 1))
+      return fail(ZODI_ERR_ARG, "per-segment mode needs seg_offsets and n_seg >= 1");
+  } else if (obs_mode != ZODI_OBS_PER_RAY) {
+    return fail(ZODI_ERR_ARG, "unknown obs_mode %d", obs_mode);
+  }
+  if (model_host->n_params < 1 || model_host->n_params > ZODI_MAX_PARAMS)
+    return fail(ZODI_ERR_MODEL, "n_params %d outside [1, %d]", model_host->n_params,
+                ZODI_MAX_PARAMS);
+  return ZODI_OK;
+}
+
+AdjKernel pick_checked(const zodi_model* model_host, const AdjPlan& plan, bool jac,
+                       zodi_status& st) {
+  AdjKernel k = pick_adjoint(model_host, plan.chunk, plan.split, jac);
+  if (k == nullptr) {
+    st = fail(ZODI_ERR_MODEL,
+              "no adjoint for this component layout (supported: cloud, 3 bands[, ring, "
+              "feature] with the Kelsall integrand)");
+    return nullptr;
+  }
+  st = validate_grad_layout(model_host);
+  return st == ZODI_OK ? k : nullptr;
+}
+
+}  // namespace
+}  // namespace zodi
+
+using namespace zodi;
+
+extern "C" {
+
+int64_t zodi_adjoint_workspace(const zodi_model* model_host, int64_t n_rays, int32_t mode,
+                               const zodi_launch* launch) {
+  if (model_host == nullptr || validate_model(model_host) != ZODI_OK || n_rays < 0) return -1;
+  if (mode != ZODI_ADJ_VJP_SUM && mode != ZODI_ADJ_VJP_COMPS && mode != ZODI_ADJ_CHI2) {
+    fail(ZODI_ERR_ARG, "unknown adjoint mode %d", mode);
+    return -1;
+  }
+  const int np = model_host->n_params;
+  if (mode == ZODI_ADJ_CHI2 && jacobian_eligible(launch, np, n_rays)) {
+    AdjPlan p;
+    if (adjoint_plan(launch, n_rays, true, p) != ZODI_OK) return -1;
+    return jacobian_bytes(np, n_rays);
+  }
+  AdjPlan p;
+  if (adjoint_plan(launch, n_rays, false, p) != ZODI_OK) return -1;
+  return recompute_bytes(p, np + 1, n_rays);
+}
+
+zodi_status zodi_jacobian(const zodi_model* model_host, const zodi_model* model_dev,
+                          const double* u, int64_t n_rays, const int64_t* seg_offsets,
+                          int32_t n_seg, const double* obs, const double* earth,
+                          int32_t obs_mode, const uint32_t* beyond, double* model_out,
+                          double* jac, int64_t ld_jac, const zodi_launch* launch, void* stream) {
+  zodi_status st = check_rays(model_host, model_dev, u, n_rays, seg_offsets, n_seg, obs, earth,
+                              obs_mode, beyond);
+  if (st != ZODI_OK) return st;
+  if (n_rays > 0 && (model_out == nullptr || jac == nullptr))
+    return fail(ZODI_ERR_ARG, "null pointer (model_out or jac)");
+  if (ld_jac < n_rays)
+    return fail(ZODI_ERR_ARG, "ld_jac %lld < n_rays %lld", (long long)ld_jac, (long long)n_rays);
+  AdjPlan plan;
+  st = adjoint_plan(launch, n_rays, true, plan);
+  if (st != ZODI_OK) return st;
+  AdjKernel k = pick_checked(model_host, plan, true, st);
+  if (k == nullptr) return st;
+  if (n_rays == 0) return ZODI_OK;
+  AdjArgs a = make_args(model_dev, u, n_rays, seg_offsets, n_seg, obs, earth, obs_mode, beyond,
+                        plan, model_host->n_params);
+  a.jac = jac;
+  a.ld_jac = ld_jac;
+  a.model_vals_out = model_out;
+  hipLaunchKernelGGL(k, dim3((unsigned)plan.grid), dim3(kAdjBlock), 0, (hipStream_t)stream, a);
+  return check_hip(hipGetLastError(), "jacobian launch");
+}
+
+zodi_status zodi_adjoint(const zodi_model* model_host, const zodi_model* model_dev,
+                         const double* u, int64_t n_rays, const int64_t* seg_offsets,
+                         int32_t n_seg, const double* obs, const double* earth, int32_t obs_mode,
+                         const uint32_t* beyond, int32_t mode, const double* weights,
+                         const double* target, const double* inv_var, double inv_var_scalar,
+                         double* model_out, double* grad, double* workspace,
+                         int64_t workspace_bytes, const zodi_launch* launch, void* stream) {
+  zodi_status st = check_rays(model_host, model_dev, u, n_rays, seg_offsets, n_seg, obs, earth,
+                              obs_mode, beyond);
+  if (st != ZODI_OK) return st;
+  if (grad == nullptr || workspace == nullptr)
+    return fail(ZODI_ERR_ARG, "null pointer (grad or workspace)");
+  if (mode == ZODI_ADJ_CHI2) {
+    if (n_rays > 0 && target == nullptr) return fail(ZODI_ERR_ARG, "chi2 mode needs target");
+  } else if (mode == ZODI_ADJ_VJP_SUM || mode == ZODI_ADJ_VJP_COMPS) {
+    if (n_rays > 0 && weights == nullptr) return fail(ZODI_ERR_ARG, "vjp mode needs weights");
+  } else {
+    return fail(ZODI_ERR_ARG, "unknown adjoint mode %d", mode);
+  }
+  const int np = model_host->n_params;
+  const int np1 = np + 1;
+
+  // chi^2 with room for the Jacobian: one pass writes J and I, then J^T g.
+  if (mode == ZODI_ADJ_CHI2 && jacobian_eligible(launch, np, n_rays) &&
+      workspace_bytes >= jacobian_bytes(np, n_rays)) {
+    AdjPlan plan;
+    st = adjoint_plan(launch, n_rays, true, plan);
+    if (st != ZODI_OK) return st;
+    AdjKernel k = pick_checked(model_host, plan, true, st);
+    if (k == nullptr) return st;
+    const int64_t chunks = (n_rays + kJacRays - 1) / kJacRays;
+    double* partials = workspace + (int64_t)np * n_rays;
+    double* vals = model_out != nullptr ? model_out : partials + chunks * np1;
+    if (n_rays > 0) {
+      AdjArgs a = make_args(model_dev, u, n_rays, seg_offsets, n_seg, obs, earth, obs_mode,
+                            beyond, plan, np);
+      a.jac = workspace;
+      a.ld_jac = n_rays;
+      a.model_vals_out = vals;
+      hipLaunchKernelGGL(k, dim3((unsigned)plan.grid), dim3(kAdjBlock), 0, (hipStream_t)stream,
+                         a);
+      st = check_hip(hipGetLastError(), "jacobian launch");
+      if (st != ZODI_OK) return st;
+      const dim3 g2((unsigned)chunks, (unsigned)((np + kJacParams - 1) / kJacParams));
+      hipLaunchKernelGGL(jacobian_chi2, g2, dim3(256), 0, (hipStream_t)stream,
+                         (const double*)workspace, n_rays, np, n_rays, (const double*)vals,
+                         target, inv_var, inv_var_scalar, partials);
+      st = check_hip(hipGetLastError(), "jacobian chi2 launch");
+      if (st != ZODI_OK) return st;
+    }
+    hipLaunchKernelGGL(adjoint_reduce, dim3(np1), dim3(256), 0, (hipStream_t)stream,
+                       (const double*)partials, (int)chunks, np1, grad);
+    return check_hip(hipGetLastError(), "adjoint reduce launch");
+  }
+
+  AdjPlan plan;
+  st = adjoint_plan(launch, n_rays, false, plan);
+  if (st != ZODI_OK) return st;
+  AdjKernel k = pick_checked(model_host, plan, false, st);
+  if (k == nullptr) return st;
+  const int64_t need = recompute_bytes(plan, np1, n_rays);
+  if (workspace_bytes < need)
+    return fail(ZODI_ERR_ARG, "workspace of %lld bytes needed, %lld given", (long long)need,
+                (long long)workspace_bytes);
+  double* vals = model_out != nullptr ? model_out : workspace + plan.grid * np1;
+  if (mode == ZODI_ADJ_CHI2 && n_rays > 0) {
+    st = zodi_forward(model_host, model_dev, u, n_rays, seg_offsets, n_seg, obs, earth, obs_mode,
+                      beyond, vals, ZODI_OUT_SUM, nullptr, stream);
+    if (st != ZODI_OK) return st;
+  }
+  AdjArgs a = make_args(model_dev, u, n_rays, seg_offsets, n_seg, obs, earth, obs_mode, beyond,
+                        plan, np);
   a.weights = weights;
   a.target = target;
   a.inv_var = inv_var;
   a.inv_var_scalar = inv_var_scalar;
   a.model_vals = vals;
   a.partials = workspace;
-  a.n_params = model_host->n_params;
   a.mode = mode;
   const size_t lds = (size_t)np1 * kAdjWaves * sizeof(double);
   hipLaunchKernelGGL(k, dim3((unsigned)plan.grid), dim3(kAdjBlock), lds, (hipStream_t)stream, a);

@@ -17,9 +17,13 @@ recomputes the integrand instead of storing node values:
 * ``emission(theta, problem)`` -- a ``torch.autograd.Function``: forward is the
   line-of-sight integral, backward is one adjoint launch with the incoming
   gradient as weights;
-* ``chi2(theta, problem, data, inv_var)`` -- one library call: the forward
-  kernel writes the model values, then one adjoint launch forms chi^2 and
-  d(chi^2)/d(theta); backward scales the stored gradient.
+* ``chi2(theta, problem, data, inv_var)`` -- one library call: one pass writes
+  the model values and the per-ray Jacobian into the workspace and a second
+  forms chi^2 and J^T g (or, with ``store_jacobian=False`` or a Jacobian over
+  72 GiB, the forward kernel writes the model values and one adjoint launch
+  forms chi^2 and d(chi^2)/d(theta)); backward scales the stored gradient;
+* ``LineOfSightProblem.jacobian(theta)`` -- the model values and the per-ray
+  Jacobian dI_r/dtheta_p themselves, for Gauss-Newton or Fisher-matrix work.
 
 Supported models: the Kelsall registry layouts (cloud and three bands,
 optionally followed by the ring and the feature). Others raise RuntimeError
@@ -111,10 +115,12 @@ class LineOfSightProblem:
     heliocentric ecliptic positions [AU], (3,) for one observer, (3, n_seg) with
     ``seg_offsets`` (n_seg + 1,) for observers owning consecutive ray ranges, or
     (3, n) with ``per_ray=True`` (one reference call over time-ordered data).
+    ``store_jacobian``: let chi^2 keep the per-ray Jacobian (n_params x n doubles)
+    in its workspace, which saves the separate forward pass.
     """
 
     def __init__(self, model, unit_vectors, obspos, earthpos, seg_offsets=None,
-                 per_ray: bool = False) -> None:
+                 per_ray: bool = False, store_jacobian: bool = True) -> None:
         self.space = ParameterSpace(model)
         base: DeviceModel = model.device_model
         self.device = base.device
@@ -135,6 +141,7 @@ class LineOfSightProblem:
             raise ValueError(msg)
         # Adjoint launch (None = library defaults); ``variant`` = components per pass.
         self.launch: Launch | None = None
+        self.store_jacobian = bool(store_jacobian)
         self._theta_key: bytes | None = None
         self._block: tuple[np.ndarray, torch.Tensor, PackedModel] | None = None
         self._workspace: torch.Tensor | None = None
@@ -175,8 +182,11 @@ class LineOfSightProblem:
         _lib.check(st, "zodi_forward")
         return out
 
-    def _ensure_workspace(self, host: np.ndarray) -> torch.Tensor:
-        need = int(self.lib.zodi_adjoint_workspace(_lib.host_ptr(host), self.n_rays,
+    def _ensure_workspace(self, host: np.ndarray, mode: int) -> tuple[torch.Tensor, int]:
+        """Workspace for ``mode`` and the byte count to pass (which selects the chi^2 path)."""
+        if mode == ADJ_CHI2 and not self.store_jacobian:
+            mode = ADJ_VJP_SUM  # the recompute path's size
+        need = int(self.lib.zodi_adjoint_workspace(_lib.host_ptr(host), self.n_rays, mode,
                                                    self.launch))
         if need < 0:
             err = self.lib.zodi_last_error().decode(errors="replace")
@@ -184,13 +194,14 @@ class LineOfSightProblem:
             raise RuntimeError(msg)
         n = (need + 7) // 8
         if self._workspace is None or self._workspace.numel() < n:
+            self._workspace = None
             self._workspace = torch.empty(max(n, 1), dtype=torch.float64, device=self.device)
-        return self._workspace
+        return self._workspace, need
 
     def _adjoint(self, theta, mode: int, weights=None, target=None, inv_var=None,
                  inv_var_scalar: float = 1.0, model_out=None) -> torch.Tensor:
         host, dev, _ = self._model(theta)
-        ws = self._ensure_workspace(host)
+        ws, need = self._ensure_workspace(host, mode)
         grad = torch.empty(self.n_params + 1, dtype=torch.float64, device=self.device)
 
         def ptr(t):
@@ -199,9 +210,20 @@ class LineOfSightProblem:
         st = self.lib.zodi_adjoint(
             _lib.host_ptr(host), dev.data_ptr(), *self._ptr_args(), mode, ptr(weights),
             ptr(target), ptr(inv_var), float(inv_var_scalar), ptr(model_out), grad.data_ptr(),
-            ws.data_ptr(), ws.numel() * 8, self.launch, _stream())
+            ws.data_ptr(), need, self.launch, _stream())
         _lib.check(st, "zodi_adjoint")
         return grad
+
+    def jacobian(self, theta) -> tuple[torch.Tensor, torch.Tensor]:
+        """(I (n,), J (n_params, n)) with J[p, r] = dI_r/dtheta_p, on the device."""
+        host, dev, _ = self._model(theta)
+        model = torch.empty(self.n_rays, dtype=torch.float64, device=self.device)
+        jac = torch.empty((self.n_params, self.n_rays), dtype=torch.float64, device=self.device)
+        st = self.lib.zodi_jacobian(_lib.host_ptr(host), dev.data_ptr(), *self._ptr_args(),
+                                    model.data_ptr(), jac.data_ptr(), self.n_rays, self.launch,
+                                    _stream())
+        _lib.check(st, "zodi_jacobian")
+        return model, jac
 
     def _rows(self, t, shape, name) -> torch.Tensor:
         t = torch.as_tensor(t, dtype=torch.float64, device=self.device)
@@ -219,7 +241,7 @@ class LineOfSightProblem:
         return self._adjoint(theta, ADJ_VJP_SUM, weights=w)[:-1]
 
     def chi2_and_grad(self, theta, data, inv_var=None, return_model: bool = False):
-        """chi^2 = sum_r (I_r - data_r)^2 inv_var_r and its gradient, in one launch.
+        """chi^2 = sum_r (I_r - data_r)^2 inv_var_r and its gradient, in one library call.
 
         ``inv_var``: None (1), a scalar, or (n,) per ray. Returns (chi2 (0-d
         tensor), grad (n_params,)[, model (n,)]) on the device.

@@ -45,7 +45,7 @@ from zodipy_amd.cutoffs import COMPONENT_CUTOFFS
 from zodipy_amd.spectral import phase_normalization
 from zodipy_amd.zodiacal_light_model import Integrand, ZodiacalLightModel
 
-ABI_VERSION = 1
+ABI_VERSION = 2
 MAX_COMPS = 16
 MAX_GROUPS = 16
 MAX_NODES = 512
