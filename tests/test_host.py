@@ -148,8 +148,11 @@ def test_adjoint_workspace_size():
     for mode in (_lib.ADJ_VJP_SUM, _lib.ADJ_VJP_COMPS):
         assert ws(blk.ctypes.data, n, mode, None) == (512 * 79 + n) * 8
     assert ws(blk.ctypes.data, 64, 0, None) == (1 * 79 + 64) * 8
-    # chi^2 stores the Jacobian [78][n], the J^T g partials of 8192-ray chunks, then I
-    chunks = -(-n // 8192)
+    # chi^2 stores the Jacobian [78][n], the J^T g partials of its ray chunks (about
+    # n / 1024 rays each, a multiple of 256 in [256, 8192]), then I
+    chunks = -(-n // 1024)  # 977 rays -> 1024 per chunk
+    assert ws(blk.ctypes.data, 196608, _lib.ADJ_CHI2, None) == (78 * 196608 + 768 * 79
+                                                                + 196608) * 8
     assert ws(blk.ctypes.data, n, _lib.ADJ_CHI2, None) == (78 * n + chunks * 79 + n) * 8
     # ... unless the launch asks for lanes sharing a ray, or J would pass 72 GiB
     two = ct.byref(_lib.Launch(0, 2, 0, 0))

@@ -395,16 +395,25 @@ __global__ __launch_bounds__(256) void adjoint_reduce(const double* __restrict__
 
 // chi^2 and its gradient from a stored Jacobian: per ray g_r = 2 (I_r - d_r) w_r
 // with w_r = 1/sigma_r^2, grad_p = sum_r g_r J[p][r], chi^2 = sum_r (I_r - d_r)^2 w_r.
-// Block (x, y) takes rays [x R, x R + R) (R = kJacRays) and parameters
-// [y P, y P + P): thread t sums rays t, t + 256, ... of the range in order, each
-// wave sums its lanes with the xor butterfly, and the waves' sums are added in
-// wave order into partials[x][p] (entry n_params: chi^2, from the y = 0 blocks).
-// adjoint_reduce then adds the partials of all x in a fixed order.
+// Block (x, y) takes rays [x R, x R + R) (R from jacobian_chunk_rays) and
+// parameters [y P, y P + P): thread t sums rays t, t + 256, ... of the range in
+// order, each wave sums its lanes with the xor butterfly, and the waves' sums are
+// added in wave order into partials[x][p] (entry n_params: chi^2, from the y = 0
+// blocks). adjoint_reduce then adds the partials of all x in a fixed order.
 constexpr int kJacParams = 32;
-constexpr int64_t kJacRays = 8192;
+
+// Rays per block of jacobian_chi2: about 1024 blocks per parameter tile (enough
+// loads in flight to stream J at full rate), 256 to 8192 rays each. A function of
+// n_rays alone, so the order of the additions does not depend on the device.
+__host__ __device__ inline int64_t jacobian_chunk_rays(int64_t n_rays) {
+  int64_t r = (n_rays + 1023) / 1024;
+  r = (r + 255) / 256 * 256;
+  return r < 256 ? 256 : (r > 8192 ? 8192 : r);
+}
 
 __global__ __launch_bounds__(256) void jacobian_chi2(const double* __restrict__ jac, int64_t ld,
                                                      int n_params, int64_t n_rays,
+                                                     int64_t chunk_rays,
                                                      const double* __restrict__ vals,
                                                      const double* __restrict__ target,
                                                      const double* __restrict__ inv_var,
@@ -415,8 +424,8 @@ __global__ __launch_bounds__(256) void jacobian_chi2(const double* __restrict__ 
   const int wave = t / kWave, lane = t % kWave;
   const int p0 = blockIdx.y * kJacParams;
   const int np = min(kJacParams, n_params - p0);
-  const int64_t r0 = (int64_t)blockIdx.x * kJacRays;
-  const int64_t r1 = min(n_rays, r0 + kJacRays);
+  const int64_t r0 = (int64_t)blockIdx.x * chunk_rays;
+  const int64_t r1 = min(n_rays, r0 + chunk_rays);
   double acc[kJacParams];
 #pragma unroll
   for (int q = 0; q < kJacParams; ++q) acc[q] = 0.0;
@@ -613,7 +622,8 @@ int64_t recompute_bytes(const AdjPlan& p, int np1, int64_t n_rays) {
 // Workspace of the chi^2 mode's Jacobian path: J [n_params][n_rays], the
 // partials of jacobian_chi2, then I.
 int64_t jacobian_bytes(int np, int64_t n_rays) {
-  const int64_t chunks = (n_rays + kJacRays - 1) / kJacRays;
+  const int64_t R = jacobian_chunk_rays(n_rays);
+  const int64_t chunks = (n_rays + R - 1) / R;
   return ((int64_t)np * n_rays + chunks * (np + 1) + n_rays) * (int64_t)sizeof(double);
 }
 
@@ -764,7 +774,8 @@ zodi_status zodi_adjoint(const zodi_model* model_host, const zodi_model* model_d
     if (st != ZODI_OK) return st;
     AdjKernel k = pick_checked(model_host, plan, true, st);
     if (k == nullptr) return st;
-    const int64_t chunks = (n_rays + kJacRays - 1) / kJacRays;
+    const int64_t chunk_rays = jacobian_chunk_rays(n_rays);
+    const int64_t chunks = (n_rays + chunk_rays - 1) / chunk_rays;
     double* partials = workspace + (int64_t)np * n_rays;
     double* vals = model_out != nullptr ? model_out : partials + chunks * np1;
     if (n_rays > 0) {
@@ -779,8 +790,8 @@ zodi_status zodi_adjoint(const zodi_model* model_host, const zodi_model* model_d
       if (st != ZODI_OK) return st;
       const dim3 g2((unsigned)chunks, (unsigned)((np + kJacParams - 1) / kJacParams));
       hipLaunchKernelGGL(jacobian_chi2, g2, dim3(256), 0, (hipStream_t)stream,
-                         (const double*)workspace, n_rays, np, n_rays, (const double*)vals,
-                         target, inv_var, inv_var_scalar, partials);
+                         (const double*)workspace, n_rays, np, n_rays, chunk_rays,
+                         (const double*)vals, target, inv_var, inv_var_scalar, partials);
       st = check_hip(hipGetLastError(), "jacobian chi2 launch");
       if (st != ZODI_OK) return st;
     }
