@@ -26,6 +26,9 @@
  *                         parameters), for parameter fitting
  *   zodi_jacobian         no reference counterpart: the per-ray Jacobian of the
  *                         same map, for Gauss-Newton and Fisher-matrix work
+ *   zodi_spectral_setup   zodipy/unpack_model.py:23-173 and zodipy/blackbody.py:33-49
+ *                         (effective spectral parameters and the Planck table,
+ *                         written into a device model block)
  */
 #ifndef ZODI_HIP_H
 #define ZODI_HIP_H
@@ -315,6 +318,40 @@ zodi_status zodi_adjoint(const zodi_model* model_host, const zodi_model* model_d
                          double inv_var_scalar, double* model_out, double* grad,
                          double* workspace, int64_t workspace_bytes, const zodi_launch* launch,
                          void* stream);
+
+/* Targets of zodi_spectral_setup; component targets carry the component index
+ * in the low byte (e.g. ZODI_SPEC_EMISSIVITY | 2). */
+enum {
+  ZODI_SPEC_EMISSIVITY = 0x100,
+  ZODI_SPEC_ALBEDO = 0x200,
+  ZODI_SPEC_SOLAR = 0x300,
+  ZODI_SPEC_C1 = 0x301,
+  ZODI_SPEC_C2 = 0x302,
+  ZODI_SPEC_C3 = 0x303,
+  ZODI_SPEC_CALIBRATION = 0x304 /* RRM; the MJy/AU -> Jy/cm factor goes into A */
+};
+
+/*
+ * zodi_spectral_setup — writes effective spectral parameters and the Planck
+ * table into a device model block, replacing zodipy/unpack_model.py:23-173
+ * (interpolation of the spectral tables at the model's frequencies and the
+ * bandpass trapezoid) and zodipy/blackbody.py:33-49 (the blackbody table) for
+ * fits that move the tables. For each of n_tab targets, effective value =
+ * sum_j A[t][j] tables[t][j] (A, tables: [n_tab][n_grid] device memory; A holds
+ * the interpolation and trapezoid weights, the table's derivative), stored in
+ * the target's slot; the derived slots (1 - albedo, (1 - albedo) emissivity,
+ * the phase normalisation and dD/dC3) are then recomputed for the whole block.
+ * With n_freq > 0 the table becomes B(T_k) = sum_i q[i] B_nu[i](T_k) in MJy/sr
+ * at the block's 100 temperatures (nu_hz, q: [n_freq] device memory), with its
+ * slopes. target is a host array. An albedo target needs the component packed
+ * with its scattering branch (ZODI_CF_SCATTER). model_host supplies the block's
+ * structure; its spectral values are not read. No reference counterpart for the
+ * in-place update itself.
+ */
+zodi_status zodi_spectral_setup(const zodi_model* model_host, zodi_model* model_dev,
+                                int32_t n_tab, int32_t n_grid, const double* A,
+                                const double* tables, const int32_t* target, int32_t n_freq,
+                                const double* nu_hz, const double* q, void* stream);
 
 /*
  * zodi_jacobian — the per-ray Jacobian jac[p * ld_jac + r] = dI_r/dtheta_p

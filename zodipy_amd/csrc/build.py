@@ -17,7 +17,8 @@ PKG = CSRC.parent
 ROOT = PKG.parent
 INCLUDE = ROOT / "include"
 LIB = PKG / "libzodi_hip.so"
-SOURCES = [CSRC / "zodi_forward.hip", CSRC / "zodi_adjoint.hip", CSRC / "zodi_pointing.hip"]
+SOURCES = [CSRC / "zodi_forward.hip", CSRC / "zodi_adjoint.hip", CSRC / "zodi_pointing.hip",
+           CSRC / "zodi_spectral.hip"]
 HEADERS = [
     CSRC / "zodi_common.hpp",
     CSRC / "zodi_device.hpp",

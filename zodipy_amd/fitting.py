@@ -43,7 +43,8 @@ from zodipy_amd.packing import PackedModel, pack_model, param_layout, param_vect
 from zodipy_amd.runtime import DeviceModel, _f64_rows, _stream
 from zodipy_amd.zodiacal_light_model import Integrand
 
-__all__ = ["ParameterSpace", "LineOfSightProblem", "emission", "chi2"]
+__all__ = ["ParameterSpace", "LineOfSightProblem", "SpectralTableFit", "emission", "chi2",
+           "chi2_tables"]
 
 
 class ParameterSpace:
@@ -58,6 +59,9 @@ class ParameterSpace:
         self._spectral = model._spectral
         self._b_nu_table = model._b_nu_table
         self._gl_x, self._gl_w = model._gl_x, model._gl_w
+        self._x = model._x
+        self._normalized_weights = model._normalized_weights
+        self._extrapolate = model._extrapolate
         self.layout = param_layout(self._ipd)
         self.names: tuple[str, ...] = tuple(self.layout.names)
         self.theta0 = param_vector(self._ipd, self._spectral)
@@ -78,8 +82,8 @@ class ParameterSpace:
             raise ValueError(msg)
         return theta
 
-    def pack(self, theta) -> PackedModel:
-        """Packed device block of the model at ``theta``."""
+    def pack(self, theta, force_scatter=()) -> PackedModel:
+        """Packed device block of the model at ``theta`` (``force_scatter``: see pack_model)."""
         theta = self._check(theta)
         lay = self.layout
         comps = {}
@@ -105,7 +109,8 @@ class ParameterSpace:
                 spec["comp"][lab]["T_0"] = float(theta[lay.spec_a[c]])
                 spec["comp"][lab]["delta"] = float(theta[lay.spec_b[c]])
             sh["calibration"] = float(theta[lay.shared[0]])
-        return pack_model(ipd, spec, self._b_nu_table, self._gl_x, self._gl_w)
+        return pack_model(ipd, spec, self._b_nu_table, self._gl_x, self._gl_w,
+                          force_scatter=force_scatter)
 
 
 class LineOfSightProblem:
@@ -171,9 +176,12 @@ class LineOfSightProblem:
                 self.seg.data_ptr() if self.seg is not None else None, self.n_seg,
                 self.obs.data_ptr(), self.earth.data_ptr(), self.mode, self.beyond.data_ptr())
 
-    def forward(self, theta, return_comps: bool = False) -> torch.Tensor:
-        """Integrated emission [MJy/sr]: (n_comps, n) or (n,)."""
-        host, dev, _ = self._model(theta)
+    def forward(self, theta, return_comps: bool = False, blocks=None) -> torch.Tensor:
+        """Integrated emission [MJy/sr]: (n_comps, n) or (n,).
+
+        ``blocks``: (host block, device block) to evaluate instead of theta's.
+        """
+        host, dev = blocks if blocks is not None else self._model(theta)[:2]
         out_mode = OUT_COMPS if return_comps else OUT_SUM
         shape = (self.n_comps, self.n_rays) if return_comps else (self.n_rays,)
         out = torch.empty(shape, dtype=torch.float64, device=self.device)
@@ -199,8 +207,8 @@ class LineOfSightProblem:
         return self._workspace, need
 
     def _adjoint(self, theta, mode: int, weights=None, target=None, inv_var=None,
-                 inv_var_scalar: float = 1.0, model_out=None) -> torch.Tensor:
-        host, dev, _ = self._model(theta)
+                 inv_var_scalar: float = 1.0, model_out=None, blocks=None) -> torch.Tensor:
+        host, dev = blocks if blocks is not None else self._model(theta)[:2]
         ws, need = self._ensure_workspace(host, mode)
         grad = torch.empty(self.n_params + 1, dtype=torch.float64, device=self.device)
 
@@ -240,11 +248,13 @@ class LineOfSightProblem:
         w = self._rows(weights, (self.n_rays,), "weights")
         return self._adjoint(theta, ADJ_VJP_SUM, weights=w)[:-1]
 
-    def chi2_and_grad(self, theta, data, inv_var=None, return_model: bool = False):
+    def chi2_and_grad(self, theta, data, inv_var=None, return_model: bool = False,
+                      blocks=None):
         """chi^2 = sum_r (I_r - data_r)^2 inv_var_r and its gradient, in one library call.
 
         ``inv_var``: None (1), a scalar, or (n,) per ray. Returns (chi2 (0-d
-        tensor), grad (n_params,)[, model (n,)]) on the device.
+        tensor), grad (n_params,)[, model (n,)]) on the device. ``blocks``: as
+        in ``forward``.
         """
         d = self._rows(data, (self.n_rays,), "data")
         iv_t, iv_s = None, 1.0
@@ -258,10 +268,94 @@ class LineOfSightProblem:
         model = (torch.empty(self.n_rays, dtype=torch.float64, device=self.device)
                  if return_model else None)
         g = self._adjoint(theta, ADJ_CHI2, target=d, inv_var=iv_t, inv_var_scalar=iv_s,
-                          model_out=model)
+                          model_out=model, blocks=blocks)
         if return_model:
             return g[-1], g[:-1], model
         return g[-1], g[:-1]
+
+
+class SpectralTableFit:
+    """chi^2 fits of a model's spectral tables, with every step on the device.
+
+    The effective spectral parameters are linear in the tables for a fixed
+    bandpass (zodipy/unpack_model.py:23-173): effective = A @ table, A holding
+    the interpolation and trapezoid weights. The fit's parameter vector is every
+    entry of every tabulated spectral parameter (``names``: ``emissivity.cloud[0]``
+    .. in ``spectral.spectral_tables`` order); everything else stays at
+    ``theta`` (ParameterSpace's effective vector; default theta0). Per call,
+    ``zodi_spectral_setup`` writes A @ table and its derived slots into the fit's
+    own device block, the chi^2 mode of ``zodi_adjoint`` gives d chi^2/d(effective
+    values), and the chain rule d chi^2/d table[t, j] = A[t, j] d chi^2/d eff_t
+    runs on the device. Components with albedo tables are packed with their
+    scattering branch, so an albedo may leave 0.
+    """
+
+    def __init__(self, problem: LineOfSightProblem, theta=None) -> None:
+        from zodipy_amd.spectral import planck_coefficients, spectral_tables
+
+        self.problem = problem
+        space = problem.space
+        self.theta = space.theta0.copy() if theta is None else space._check(theta)
+        self.tables = spectral_tables(space._ipd, space._x, space._normalized_weights,
+                                      space._extrapolate)
+        if not self.tables:
+            msg = "the model has no spectral tables"
+            raise ValueError(msg)
+        self.n_grid = int(self.tables[0].values.size)
+        self.names = tuple(f"{t.name}[{j}]" for t in self.tables for j in range(self.n_grid))
+        dev = problem.device
+        self._A = torch.as_tensor(np.stack([t.weights for t in self.tables]), device=dev)
+        self.values0 = np.concatenate([t.values for t in self.tables])
+        self._targets = np.array([t.target for t in self.tables], dtype=np.int32)
+        self._eff_index = torch.as_tensor([space.index(t.name) for t in self.tables],
+                                          device=dev)
+        scatter = tuple(lab for lab in space._ipd.comps
+                        if any(t.name == f"albedo.{lab.value}" for t in self.tables))
+        packed = space.pack(self.theta, force_scatter=scatter)
+        self._host = np.ascontiguousarray(packed.block)
+        self._dev = torch.from_numpy(self._host.view(np.uint8).copy()).to(dev)
+        nu, q = planck_coefficients(space._x, space._normalized_weights)
+        self._nu = torch.as_tensor(nu, device=dev)
+        self._q = torch.as_tensor(q, device=dev)
+
+    @property
+    def n_params(self) -> int:
+        return len(self.names)
+
+    def set_tables(self, values, planck: bool = False) -> None:
+        """Write the effective values of ``values`` (n_params,) into the device block.
+
+        ``planck``: also rebuild the block's Planck table for the model's bandpass
+        on the device (it does not depend on the tables).
+        """
+        v = torch.as_tensor(values, dtype=torch.float64, device=self.problem.device)
+        if v.shape != (self.n_params,):
+            msg = f"table values must have shape ({self.n_params},), got {tuple(v.shape)}"
+            raise ValueError(msg)
+        v = v.contiguous()
+        n_freq = int(self._nu.numel()) if planck else 0
+        st = self.problem.lib.zodi_spectral_setup(
+            _lib.host_ptr(self._host), self._dev.data_ptr(), len(self.tables), self.n_grid,
+            self._A.data_ptr(), v.data_ptr(), self._targets.ctypes.data, n_freq,
+            self._nu.data_ptr() if planck else None, self._q.data_ptr() if planck else None,
+            _stream())
+        _lib.check(st, "zodi_spectral_setup")
+
+    def device_block(self) -> np.ndarray:
+        """The fit's device model block, copied to the host."""
+        return self._dev.cpu().numpy().view(self._host.dtype).copy()
+
+    def forward(self, values, return_comps: bool = False) -> torch.Tensor:
+        self.set_tables(values)
+        return self.problem.forward(None, return_comps, blocks=(self._host, self._dev))
+
+    def chi2_and_grad(self, values, data, inv_var=None):
+        """(chi^2, d chi^2 / d table entries (n_params,)) on the device."""
+        self.set_tables(values)
+        c2, g_eff = self.problem.chi2_and_grad(None, data, inv_var,
+                                               blocks=(self._host, self._dev))
+        g = self._A * g_eff[self._eff_index][:, None]
+        return c2, g.reshape(-1)
 
 
 class _Emission(torch.autograd.Function):
@@ -305,3 +399,24 @@ def emission(theta: torch.Tensor, problem: LineOfSightProblem,
 def chi2(theta: torch.Tensor, problem: LineOfSightProblem, data, inv_var=None) -> torch.Tensor:
     """Differentiable chi^2 = sum (I - data)^2 inv_var, forward and gradient fused."""
     return _Chi2.apply(theta, problem, data, inv_var)
+
+
+class _Chi2Tables(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, values, fit, data, inv_var):
+        value, grad = fit.chi2_and_grad(values.detach(), data, inv_var)
+        ctx.save_for_backward(grad)
+        ctx.meta = (values.device, values.dtype)
+        return value
+
+    @staticmethod
+    def backward(ctx, grad_out):
+        (grad,) = ctx.saved_tensors
+        device, dtype = ctx.meta
+        g = grad * grad_out.to(device=grad.device, dtype=torch.float64)
+        return g.to(device=device, dtype=dtype), None, None, None
+
+
+def chi2_tables(values: torch.Tensor, fit: SpectralTableFit, data, inv_var=None) -> torch.Tensor:
+    """Differentiable chi^2 as a function of the spectral table entries."""
+    return _Chi2Tables.apply(values, fit, data, inv_var)

@@ -273,8 +273,13 @@ class PackedModel:
 
 
 def pack_model(model: ZodiacalLightModel, spectral: dict, b_nu_table: np.ndarray,
-               gl_x: np.ndarray, gl_w: np.ndarray) -> PackedModel:
-    """Build the device block for ``model`` with effective ``spectral`` parameters."""
+               gl_x: np.ndarray, gl_w: np.ndarray, force_scatter=()) -> PackedModel:
+    """Build the device block for ``model`` with effective ``spectral`` parameters.
+
+    ``force_scatter``: labels of components whose scattering branch is kept even at
+    albedo 0 (their albedo may change on the device; a zero albedo then adds an
+    exact zero).
+    """
     labels = list(model.comps.keys())
     n_comps = len(labels)
     if not 1 <= n_comps <= MAX_COMPS:
@@ -348,7 +353,7 @@ def pack_model(model: ZodiacalLightModel, spectral: dict, b_nu_table: np.ndarray
             slots[P_ALBEDO] = sp["albedo"]
             slots[P_ONE_MINUS_ALBEDO] = 1 - sp["albedo"]
             slots[P_SCALE] = (1 - sp["albedo"]) * sp["emissivity"]
-            if sp["albedo"] != 0:
+            if sp["albedo"] != 0 or label in force_scatter:
                 flags |= CF_SCATTER
             temps_law.append((float(shared["T_0"]), float(shared["delta"])))
         else:

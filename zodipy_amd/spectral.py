@@ -24,6 +24,8 @@ used only for the trapezoid rule.
 
 from __future__ import annotations
 
+from dataclasses import dataclass
+
 import numpy as np
 from scipy.integrate import trapezoid
 
@@ -168,3 +170,98 @@ def phase_normalization(C1: float, C2: float, C3: float) -> float:
     (zodipy/scattering.py:53-59).
     """
     return 1.0 / (2 * np.pi * (2 * C1 + np.pi * C2 + (np.exp(C3 * np.pi) + 1) / (C3**2 + 1)))
+
+
+# ----------------------------------------------------------- tabulated entries
+# Targets of zodi_spectral_setup (include/zodi_hip.h): component slots carry the
+# component index in the low byte.
+SPEC_EMISSIVITY = 0x100
+SPEC_ALBEDO = 0x200
+SPEC_SOLAR, SPEC_C1, SPEC_C2, SPEC_C3, SPEC_CALIBRATION = 0x300, 0x301, 0x302, 0x303, 0x304
+
+
+def effective_weights(x_model: np.ndarray, weights, spectrum: u.Quantity, n: int,
+                      nearest: bool = False, extrapolate: bool = False) -> np.ndarray:
+    """Weights w over a table's own entries such that the effective parameter is w @ table.
+
+    ``effective_param`` is linear in the table values once the frequencies are
+    fixed (linear or nearest interpolation, then the trapezoid over the
+    bandpass), so w_j is its value on the j-th unit table; w therefore follows
+    every rounding and ordering rule of the unpacking itself.
+    """
+    eye = np.eye(n)
+    return np.array([float(effective_param(x_model, weights, spectrum, eye[j], nearest=nearest,
+                                           extrapolate=extrapolate)) for j in range(n)])
+
+
+@dataclass(frozen=True)
+class SpectralTable:
+    """One tabulated spectral parameter of a model and its effective-value weights.
+
+    ``name``: parameter name as in the gradient vector (``emissivity.cloud``,
+    ``albedo.band1``, ``solar_irradiance``, ``C1``.., ``calibration``);
+    ``target``: its zodi_spectral_setup target code; ``values``: the table as the
+    model holds it; ``weights``: d(effective value)/d(values), the unit factor of
+    the RRM calibration included.
+    """
+
+    name: str
+    target: int
+    values: np.ndarray
+    weights: np.ndarray
+
+
+def spectral_tables(model: ZodiacalLightModel, x: u.Quantity, normalized_weights,
+                    extrapolate: bool) -> list[SpectralTable]:
+    """Every tabulated spectral parameter of ``model`` at ``x`` (zodipy/unpack_model.py:23-173).
+
+    Absent tables (no albedo, solar or phase terms in the Planck and Odegard
+    models) have no entry: their effective values stay 0.
+    """
+    xm = np.asarray(x.to_value(model.spectrum.unit, equivalencies=u.spectral()), dtype=np.float64)
+    n = len(np.asarray(model.spectrum.value))
+
+    def table(name, target, values, nearest=False, scale=1.0):
+        vals = np.asarray(values, dtype=np.float64)
+        w = effective_weights(xm, normalized_weights, model.spectrum, n, nearest=nearest,
+                              extrapolate=extrapolate) * scale
+        return SpectralTable(name, target, vals, w)
+
+    out: list[SpectralTable] = []
+    labels = list(model.comps)
+    if isinstance(model, Kelsall):
+        for c, label in enumerate(labels):
+            out.append(table(f"emissivity.{label.value}", SPEC_EMISSIVITY | c,
+                             model.emissivities[label]))
+        if model.albedos is not None:
+            for c, label in enumerate(labels):
+                out.append(table(f"albedo.{label.value}", SPEC_ALBEDO | c, model.albedos[label]))
+        if model.solar_irradiance is not None:
+            out.append(table("solar_irradiance", SPEC_SOLAR, model.solar_irradiance))
+        for key, target in (("C1", SPEC_C1), ("C2", SPEC_C2), ("C3", SPEC_C3)):
+            vals = getattr(model, key)
+            if vals is not None:
+                out.append(table(key, target, vals, nearest=True))
+        return out
+    if isinstance(model, RRM):
+        return [table("calibration", SPEC_CALIBRATION, model.calibration,
+                      scale=CALIBRATION_UNIT_FACTOR)]
+    msg = f"unsupported model type {type(model).__name__}"
+    raise TypeError(msg)
+
+
+def planck_coefficients(x: u.Quantity, normalized_weights) -> tuple[np.ndarray, np.ndarray]:
+    """(nu [Hz], q) with the Planck table B(T) = sum_i q_i B_nu_i(T) (zodipy/blackbody.py:33-49).
+
+    q holds the trapezoid coefficients over the user's x times the normalised
+    weights, or [1] for a single frequency.
+    """
+    nu = np.atleast_1d(np.asarray(x.to_value(u.Hz, equivalencies=u.spectral()), dtype=np.float64))
+    if normalized_weights is None:
+        return nu[:1], np.ones(1)
+    xv = np.asarray(x.value, dtype=np.float64)
+    d = np.diff(xv)
+    c = np.zeros_like(xv)
+    c[:-1] += d / 2.0
+    c[1:] += d / 2.0
+    return nu, c * np.asarray(normalized_weights, dtype=np.float64)
