@@ -182,19 +182,23 @@ def grad_cost(dev, reps: int = 10) -> dict:
     (a) DIRBE 25 um, HEALPix nside=128 pixel centres, observer = Earth =
         (1, 0, 0) AU: target = model at theta with every parameter perturbed by
         seeded N(0, 1%) relative noise plus N(0, 0.01 MJy/sr); loss
-        chi^2 = sum (I - d)^2 / 0.01^2. The library's chi^2 call (forward kernel,
-        then the adjoint) against the forward launch alone, both at theta0.
+        chi^2 = sum (I - d)^2 / 0.01^2. The library's chi^2 call (the Jacobian
+        pass, then J^T g) against the forward launch alone, both at theta0. The
+        same chi^2 as a function of the spectral tables (SpectralTableFit: device
+        set-up of the effective values, chi^2, chain rule) is timed beside it.
     (b) Planck-2018-style six-component model at 857 GHz, 30-frequency top-hat
         bandpass of +-10% (771.3-942.7 GHz, extrapolated past 857 GHz), 1000
         observers on a 1 AU orbit with seeded 5000-40000 pointings each: the
         chi^2 call over the whole batch against the batched forward; the
-        registry's four-component planck18 is timed the same way.
+        registry's four-component planck18 is timed the same way. The device
+        set-up of that bandpass's Planck table (30 frequencies x 100
+        temperatures) and effective spectral values is timed on its own.
     """
     import torch
 
     from zodipy_amd import Model, healpix
     from zodipy_amd import units as zu
-    from zodipy_amd.fitting import LineOfSightProblem
+    from zodipy_amd.fitting import LineOfSightProblem, SpectralTableFit
     from zodipy_amd.zodiacal_light_model import model_registry
 
     out = {}
@@ -221,6 +225,15 @@ def grad_cost(dev, reps: int = 10) -> dict:
     }
     log(f"chi2 gradient (DIRBE 25 um, nside 128, {prob.n_params} params): fwd {t_fwd:.3f} ms, "
         f"fwd+grad {t_fg:.3f} ms, ratio {t_fg / t_fwd:.3f}")
+    fit = SpectralTableFit(prob)
+    vals = torch.as_tensor(fit.values0, device=dev)  # a device-resident fit's table vector
+    fit.chi2_and_grad(vals, data, inv_var)
+    t_tab = _event_time(lambda: fit.chi2_and_grad(vals, data, inv_var), reps)
+    out["spectral_tables_dirbe25_nside128"] = {
+        "n_table_entries": fit.n_params, "chi2_grad_ms": round(t_tab, 4),
+        "ratio_vs_fwd": round(t_tab / t_fwd, 4)}
+    log(f"chi2 gradient w.r.t. {fit.n_params} spectral table entries: {t_tab:.3f} ms "
+        f"(set-up + chi2 + chain rule), ratio {t_tab / t_fwd:.3f}")
 
     name = "planck18-style-6comp"
     try:
@@ -253,6 +266,16 @@ def grad_cost(dev, reps: int = 10) -> dict:
         }
         log(f"batched chi2 gradient ({mname}, 857 GHz, {int(seg[-1])} rays): fwd {bt_fwd:.2f} ms, "
             f"fwd+grad {bt_fg:.2f} ms, ratio {bt_fg / bt_fwd:.3f}")
+        if mname == name:
+            sfit = SpectralTableFit(bprob)
+            sfit.set_tables(sfit.values0, planck=True)
+            t_set = _event_time(lambda: sfit.set_tables(sfit.values0, planck=True), reps)
+            out["spectral_setup_857GHz_tophat30"] = {
+                "n_freq": 30, "n_temps": 100, "n_tables": len(sfit.tables),
+                "ms": round(t_set, 4)}
+            log(f"spectral set-up on the device (30-frequency bandpass, {len(sfit.tables)} "
+                f"tables): {t_set * 1e3:.1f} us")
+            del sfit
         del bprob
     out["ratio"] = out["chi2_dirbe25_nside128"]["ratio"]
     return out

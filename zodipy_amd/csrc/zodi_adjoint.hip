@@ -192,7 +192,8 @@ __device__ __forceinline__ void adjoint_group(Grp<Ks...>, std::integer_sequence<
             const double wnn = wn * dg.n;
             A[5 + NP] += wnn * (oma * nd.B);                  // emissivity
             A[5 + NP + 1] += wnn * (scat_unit - eps * nd.B);  // albedo
-            const double wt = wnn * (oma * eps) * dBdT;
+            // (1 - a) eps, packed with the same rounding as oma * eps
+            const double wt = wnn * P[ZODI_P_SCALE] * dBdT;
             sh[0] += wt * dT_dT0;
             sh[1] += wt * dT_ddelta;
             const double was = wnn * alb * inv_R2;
