@@ -1,10 +1,10 @@
-"""FP64 autograd twin of the oracle's Kelsall line-of-sight integral.
+"""FP64 autograd twin of the oracle's Kelsall and RRM line-of-sight integrals.
 
 TEST INFRASTRUCTURE ONLY: the reference gradient the adjoint kernel is checked
 against. Only tests may import it; the product path never calls it.
 
 It restates the oracle functions of zodi_oracle.py (which follow
-zodipy/brightness.py:21-56, zodipy/number_density.py:47-181,
+zodipy/brightness.py:21-83, zodipy/number_density.py:47-404,
 zodipy/scattering.py:11-59, zodipy/blackbody.py:16-30 and
 zodipy/line_of_sight.py:55-105) with torch float64 operations on the CPU, so
 that torch.autograd differentiates the same arithmetic with respect to a
@@ -18,8 +18,10 @@ invisible in the values:
 theta follows the parameter order of the reference's parameter dictionary
 (zodipy/zodiacal_light_model.py:38-55: component init fields, angles in degrees)
 followed by T_0, delta, emissivity.<comp>, albedo.<comp>, solar_irradiance,
-C1, C2, C3 -- entries are addressed by these names, so the twin does not depend
-on the product's packing code.
+C1, C2, C3 (Kelsall) or T_0.<comp>, delta.<comp>, calibration (RRM) -- entries
+are addressed by these names, so the twin does not depend on the product's
+packing code. The RRM densities' masks (R_inner, R_outer) enter through
+torch.where, so their derivative is 0, as the kernels take it.
 """
 
 from __future__ import annotations
@@ -81,8 +83,40 @@ def _density(kind, X, p, earth):
         e = e + torch.abs(Z) / p["sigma_z"]
         e = e + dth**2 / (p["sigma_theta"] * _DEG) ** 2
         return p["n_0"] * torch.exp(-e)
-    msg = f"no twin for component kind {kind}"
-    raise ValueError(msg)
+    if kind in ("ring_rrm", "feature_rrm"):
+        return p["A"] * _density(kind[:-4], X, p, earth)
+    if kind == "interstellar":
+        return p["amp"] * torch.ones_like(R)
+    sin_beta = Z / R
+    beta = torch.arcsin(sin_beta)
+    if kind in ("fan", "comet"):
+        # zodipy/number_density.py:184-256
+        za = torch.abs(Z)
+        eps = torch.where(za < p["Z_0"], 2 - za / p["Z_0"], torch.ones_like(za))
+        f = torch.exp(-p["P"] * torch.sin(torch.abs(beta) ** eps))
+        if kind == "fan":
+            inside = R <= p["R_outer"]
+            val = R ** (-p["gamma"]) * torch.cos(beta) ** p["Q"] * f
+        else:
+            inside = (R >= p["R_inner"]) & (R <= p["R_outer"])
+            val = p["amp"] * R ** (-p["gamma"]) * f
+        return torch.where(inside, val, torch.zeros_like(val))
+    inside = (R >= p["R_inner"]) & (R <= p["R_outer"])
+    beta_deg = beta * (180.0 / np.pi)
+    if kind == "narrow_band":
+        # zodipy/number_density.py:267-304
+        babs = torch.abs(beta_deg)
+        f = torch.where(babs < p["beta_nb"], torch.exp(p["G"] * (babs - p["beta_nb"])),
+                        torch.zeros_like(babs))
+    elif kind == "broad_band":
+        # zodipy/number_density.py:307-342
+        f = (torch.exp(-0.5 * ((beta_deg - p["beta_bb"]) / p["sigma_bb"]) ** 2)
+             + torch.exp(-0.5 * ((beta_deg + p["beta_bb"]) / p["sigma_bb"]) ** 2))
+    else:
+        msg = f"no twin for component kind {kind}"
+        raise ValueError(msg)
+    val = p["A"] * (R / p["R_outer"]) ** (-p["gamma"]) * f
+    return torch.where(inside, val, torch.zeros_like(val))
 
 
 def _kind(label: str) -> str:
@@ -90,6 +124,8 @@ def _kind(label: str) -> str:
         return "cloud"
     if label.startswith("band"):
         return "band"
+    if label.endswith("narrow_band"):
+        return "narrow_band"
     return label
 
 
@@ -151,5 +187,50 @@ class KelsallTwin:
             em = em + alb * (sh["solar_irradiance"] / R**2) * phi
             n = _density(_kind(label), X, comps[label], Ea)
             f = em * n * 0.5 * (b - a)
+            out.append((f * w).sum(0))
+        return torch.stack(out)
+
+
+class RRMTwin:
+    """Twin of ``OracleModel`` for the RRM model, parameterised by theta.
+
+    f = B(T_c) n_c calibration (stop - start)/2 with T_c = T_0,c R^-delta_c per
+    component (zodipy/brightness.py:59-83).
+    """
+
+    def __init__(self, oracle_model: O.OracleModel, names) -> None:
+        self.om = oracle_model
+        self.names = list(names)
+        self.labels = [lab.value for lab in oracle_model.ipd_model.comps]
+        tab = oracle_model.b_nu_table
+        self.tab_T = torch.as_tensor(tab[0], dtype=torch.float64)
+        self.tab_B = torch.as_tensor(tab[1], dtype=torch.float64)
+
+    def emission(self, theta, unit_vectors, obs_xyz, earth_xyz):
+        """(ncomps, N) emission for one observer (3,) or per-ray observers (3, N)."""
+        uvec = np.asarray(unit_vectors, dtype=np.float64)
+        obs = np.asarray(obs_xyz, dtype=np.float64).reshape(3, -1)
+        earth = np.asarray(earth_xyz, dtype=np.float64).reshape(3, -1)
+        start, stop = O.get_line_of_sight_range(self.om.ipd_model.comps.keys(), uvec, obs)
+        idx = {n: k for k, n in enumerate(self.names)}
+        x = torch.as_tensor(self.om.points, dtype=torch.float64).reshape(-1, 1)
+        w = torch.as_tensor(self.om.weights, dtype=torch.float64).reshape(-1, 1)
+        U = torch.as_tensor(uvec).reshape(3, 1, -1)
+        Xo = torch.as_tensor(obs).reshape(3, 1, -1)
+        Ea = torch.as_tensor(earth)
+        cal = theta[idx["calibration"]]
+        out = []
+        for label, ipd_label in zip(self.labels, self.om.ipd_model.comps):
+            pre = f"comps.{label}."
+            p = {n[len(pre):]: theta[k] for n, k in idx.items() if n.startswith(pre)}
+            a = torch.as_tensor(start[ipd_label]).reshape(1, -1)
+            b = torch.as_tensor(stop[ipd_label]).reshape(1, -1)
+            R_los = 0.5 * (b - a) * x + 0.5 * (b + a)
+            X = R_los.unsqueeze(0) * U + Xo
+            R = torch.sqrt(X[0] ** 2 + X[1] ** 2 + X[2] ** 2)
+            T = theta[idx[f"T_0.{label}"]] * R ** (-theta[idx[f"delta.{label}"]])
+            B = _interp(T.reshape(-1), self.tab_T, self.tab_B).reshape(T.shape)
+            n = _density(_kind(label), X, p, Ea)
+            f = B * n * cal * 0.5 * (b - a)
             out.append((f * w).sum(0))
         return torch.stack(out)

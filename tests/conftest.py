@@ -29,3 +29,28 @@ def gpu():
 
 def pytest_report_header(config):
     return f"zodipy_amd root: {ROOT}; HIP lib: {os.environ.get('ZODI_HIP_LIB', 'in-tree')}"
+
+
+@pytest.fixture(scope="session")
+def cloud_ring_model():
+    """Name of a registered two-component Kelsall model (DIRBE's cloud and ring).
+
+    Its components form no registry layout, so the kernels run it through the
+    generic path and the adjoint rejects it.
+    """
+    import dataclasses
+
+    from zodipy_amd.component import ComponentLabel
+    from zodipy_amd.zodiacal_light_model import model_registry
+
+    name = "test-cloud-ring"
+    try:
+        model_registry.get_model(name)
+    except ValueError:
+        d = model_registry.get_model("dirbe")
+        keep = [ComponentLabel.CLOUD, ComponentLabel.RING]
+        model_registry.register_model(name, dataclasses.replace(
+            d, comps={k: d.comps[k] for k in keep},
+            emissivities={k: d.emissivities[k] for k in keep},
+            albedos={k: d.albedos[k] for k in keep}))
+    return name

@@ -188,7 +188,7 @@ def test_jacobian_leading_dimension_and_chunks(setup):
     ld = n + 37
     jp = torch.full((prob.n_params, ld), -7.0, dtype=torch.float64, device="cuda")
     Ip = torch.empty(n, dtype=torch.float64, device="cuda")
-    st = prob.lib.zodi_jacobian(_lib.host_ptr(host), dev.data_ptr(), *prob._ptr_args(),
+    st = prob.lib.zodi_jacobian(_lib.host_ptr(host), dev.data_ptr(), *prob._ptr_args(host),
                                 Ip.data_ptr(), jp.data_ptr(), ld, None, _stream())
     _lib.check(st, "zodi_jacobian")
     assert torch.equal(jp[:, :n], J) and torch.equal(Ip, I)
@@ -340,8 +340,8 @@ def test_empty_ray_set(gpu):
     assert I.shape == (0,) and J.shape == (prob.n_params, 0)
 
 
-def test_unsupported_layout_raises(gpu):
-    model = Model(1.25 * u.micron, name="rrm-experimental", extrapolate=True)
+def test_unsupported_layout_raises(gpu, cloud_ring_model):
+    model = Model(25 * u.micron, name=cloud_ring_model)
     v = healpix.pix2vec(2)
     prob = LineOfSightProblem(model, v, OBS, EARTH)
     with pytest.raises(RuntimeError, match="no adjoint for this component layout"):
@@ -381,3 +381,133 @@ def test_invalid_adjoint_launch_raises(gpu):
         prob.launch = bad
         with pytest.raises(RuntimeError, match=msg):
             prob.vjp(prob.space.theta0, w)
+
+
+def rrm_theta_from_oracle(om: OracleModel):
+    """(names, theta) of the RRM model from the oracle's registry model and unpacking."""
+    names, vals = [], []
+    for lab, comp in om.ipd_model.comps.items():
+        for k, v in comp.init_params().items():
+            names.append(f"comps.{lab.value}.{k}")
+            vals.append(float(v))
+    for key in ("T_0", "delta"):
+        for lab in om.ipd_model.comps:
+            names.append(f"{key}.{lab.value}")
+            vals.append(float(om.comp_params[lab][key]))
+    names.append("calibration")
+    vals.append(float(om.shared_params["calibration"]))
+    return names, np.array(vals)
+
+
+@pytest.fixture(scope="module", params=[25.0, 100.0], ids=["25um", "100um"])
+def rrm_setup(request, gpu):
+    x = request.param * u.micron
+    model = Model(x, name="rrm-experimental")
+    om = OracleModel(x, name="rrm-experimental")
+    names, theta = rrm_theta_from_oracle(om)
+    v = random_dirs(600, 21)
+    prob = LineOfSightProblem(model, v, OBS, EARTH)
+    return om, names, theta, v, prob
+
+
+def test_rrm_theta0_matches_oracle(rrm_setup):
+    _, names, theta, _, prob = rrm_setup
+    assert list(prob.space.names) == names
+    assert np.array_equal(prob.space.theta0, theta)
+
+
+def test_rrm_vjp_and_jacobian_match_twin(rrm_setup):
+    """RRM adjoint (fan, comet, narrow and broad bands, interstellar, RRM ring and
+    feature) against the RRM twin's autograd: the VJP, and Jacobian rows."""
+    from oracle.zodi_torch_twin import RRMTwin
+
+    om, names, theta, v, prob = rrm_setup
+    tw = RRMTwin(om, names)
+    w = np.random.default_rng(22).normal(size=v.shape[1])
+    g = prob.vjp(theta, torch.as_tensor(w, device="cuda")).cpu().numpy()
+    th = torch.tensor(theta, dtype=torch.float64, requires_grad=True)
+    (tw.emission(th, v, OBS, EARTH).sum(0) * torch.as_tensor(w)).sum().backward()
+    sensitivity_check(g, th.grad.numpy(), theta, 1e-8, 1e-8, "rrm vjp vs twin")
+    I, J = prob.jacobian(theta)
+    torch.testing.assert_close(I, prob.forward(theta), rtol=1e-13, atol=0)
+    rays = [3, 100, 321, 599]
+    jt = torch.autograd.functional.jacobian(
+        lambda t: tw.emission(t, v[:, rays], OBS, EARTH).sum(0),
+        torch.tensor(theta, dtype=torch.float64))
+    Jc = J.cpu().numpy()
+    for k, r in enumerate(rays):
+        sensitivity_check(Jc[:, r], jt[k].numpy(), theta, 1e-8, 1e-8, f"rrm J[:, {r}] vs twin")
+
+
+def test_rrm_vjp_matches_central_fd(rrm_setup):
+    """Central differences of the weighted sum, differenced per component before
+    summing (the RRM components span orders of magnitude, and a weak one's change
+    would otherwise sink in the rounding of the strong ones). The temperature
+    laws' entries (T_0.*, delta.*) take a 1e-9 step: B(T) is tabulated piecewise
+    linearly (5.15 K apart), and a step of 1e-6 T_0 (3e-4 K) lets a few of the
+    30 000 ray-node temperatures straddle a table node, where the difference
+    quotient averages two slopes."""
+    _, names, theta, v, prob = rrm_setup
+    w = torch.as_tensor(np.random.default_rng(23).normal(size=v.shape[1]), device="cuda")
+    g = prob.vjp(theta, w).cpu().numpy()
+    fd = np.empty_like(theta)
+    for i in range(theta.size):
+        rel = 1e-9 if names[i].startswith(("T_0.", "delta.")) else 1e-6
+        h = rel * max(abs(theta[i]), 1e-3)
+        tp, tm = theta.copy(), theta.copy()
+        tp[i] += h
+        tm[i] -= h
+        d = prob.forward(tp, return_comps=True) - prob.forward(tm, return_comps=True)
+        fd[i] = float((d * w).sum()) / (2 * h)
+    sensitivity_check(g, fd, theta, 1e-6, 1e-6, "rrm vjp vs central FD")
+
+
+def test_rrm_chi2_paths_agree(rrm_setup):
+    _, _, theta, v, prob = rrm_setup
+    I0 = prob.forward(theta)
+    data = I0 * torch.as_tensor(1 + 0.01 * np.random.default_rng(24).normal(size=v.shape[1]),
+                                device="cuda")
+    c_j, g_j = prob.chi2_and_grad(theta, data, 3.0)
+    prob.store_jacobian = False
+    try:
+        c_r, g_r = prob.chi2_and_grad(theta, data, 3.0)
+    finally:
+        prob.store_jacobian = True
+    assert abs(float(c_j) - float(c_r)) <= 1e-11 * float(c_r)
+    res = (I0 - data).cpu()
+    g_v = prob.vjp(theta, (2 * 3.0 * res).cuda())
+    sensitivity_check(g_r.cpu().numpy(), g_v.cpu().numpy(), theta, 1e-10, 1e-10,
+                      "rrm chi2 (recompute) vs vjp")
+    sensitivity_check(g_j.cpu().numpy(), g_r.cpu().numpy(), theta, 1e-8, 1e-9,
+                      "rrm chi2 jacobian vs recompute")
+
+
+def test_separated_temperature_laws_match_oracle(gpu):
+    """RRM parameter vectors whose narrow bands no longer share a temperature law.
+    ParameterSpace packs every RRM component in a group of its own (8 groups,
+    where the front end merges the bands into 6), so the structure stays fixed and
+    the observers' cutoff bits follow the evaluated block's groups: forward and
+    gradient agree with the oracle and its twin."""
+    from oracle.zodi_torch_twin import RRMTwin
+
+    x = 25 * u.micron
+    model = Model(x, name="rrm-experimental")
+    om = OracleModel(x, name="rrm-experimental")
+    names, theta = rrm_theta_from_oracle(om)
+    v = random_dirs(300, 25)
+    prob = LineOfSightProblem(model, v, OBS, EARTH)
+    t = theta.copy()
+    t[names.index("T_0.inner_narrow_band")] *= 1.01
+    assert prob.space.pack(t).n_groups == prob.space.pack(theta).n_groups == 8
+    assert model._packed.n_groups == 6
+    for label in om.ipd_model.comps:
+        if label.value == "inner_narrow_band":
+            om.comp_params[label]["T_0"] = t[names.index("T_0.inner_narrow_band")]
+    got = prob.forward(t, return_comps=True).cpu().numpy()
+    np.testing.assert_allclose(got, om.evaluate_vectors(v, OBS, EARTH), rtol=1e-10, atol=0)
+    w = np.random.default_rng(26).normal(size=v.shape[1])
+    g = prob.vjp(t, torch.as_tensor(w, device="cuda")).cpu().numpy()
+    tw = RRMTwin(OracleModel(x, name="rrm-experimental"), names)
+    th = torch.tensor(t, dtype=torch.float64, requires_grad=True)
+    (tw.emission(th, v, OBS, EARTH).sum(0) * torch.as_tensor(w)).sum().backward()
+    sensitivity_check(g, th.grad.numpy(), t, 1e-8, 1e-8, "regrouped vjp vs twin")

@@ -115,7 +115,7 @@ def test_host_validation_errors():
 
 
 @cpu_only
-def test_adjoint_host_validation_errors():
+def test_adjoint_host_validation_errors(cloud_ring_model):
     lib = _lib.load()
     blk = np.ascontiguousarray(_packed().block)
     assert _call_adjoint(lib, blk, mode=7) == 1
@@ -132,9 +132,8 @@ def test_adjoint_host_validation_errors():
     corrupt = blk.copy()
     corrupt["grad_shared"][0, 5] = -1
     assert _call_adjoint(lib, corrupt) == 2
-    rrm = np.ascontiguousarray(_packed("rrm-experimental", 1.25 * u.micron,
-                                       extrapolate=True).block)
-    assert _call_adjoint(lib, rrm) == 2
+    odd = np.ascontiguousarray(_packed(cloud_ring_model).block)
+    assert _call_adjoint(lib, odd) == 2
     assert b"no adjoint" in lib.zodi_last_error()
 
 
@@ -200,7 +199,7 @@ def test_spectral_setup_host_validation_errors():
 
 
 @cpu_only
-def test_jacobian_host_validation_errors():
+def test_jacobian_host_validation_errors(cloud_ring_model):
     lib = _lib.load()
     blk = np.ascontiguousarray(_packed().block)
     d = ct.c_void_p(0x1000)  # never dereferenced: every case fails host validation
@@ -215,9 +214,8 @@ def test_jacobian_host_validation_errors():
     assert call(blk, jac=None) == 1
     assert call(blk, launch=ct.byref(_lib.Launch(0, 2, 0, 0))) == 1
     assert b"one lane per ray" in lib.zodi_last_error()
-    rrm = np.ascontiguousarray(_packed("rrm-experimental", 1.25 * u.micron,
-                                       extrapolate=True).block)
-    assert call(rrm) == 2
+    odd = np.ascontiguousarray(_packed(cloud_ring_model).block)
+    assert call(odd) == 2
     assert call(blk, n=0, out=None, jac=None, ld=0) == 0  # nothing to do
 
 

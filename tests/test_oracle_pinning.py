@@ -357,3 +357,72 @@ def test_torch_twin_gradient_matches_oracle_central_differences():
         fm = (_oracle_with_theta(om, names, tm, lam).evaluate_vectors(v, obs, obs).sum(0) * wts).sum()
         fd = (fp - fm) / (2 * h)
         assert g[i] == pytest.approx(fd, rel=2e-6, abs=1e-9 * np.abs(g).max()), names[i]
+
+
+def _rrm_theta(om):
+    names, vals = [], []
+    for lab, comp in om.ipd_model.comps.items():
+        for k, val in comp.init_params().items():
+            names.append(f"comps.{lab.value}.{k}")
+            vals.append(float(val))
+    for key in ("T_0", "delta"):
+        for lab in om.ipd_model.comps:
+            names.append(f"{key}.{lab.value}")
+            vals.append(float(om.comp_params[lab][key]))
+    names.append("calibration")
+    vals.append(float(om.shared_params["calibration"]))
+    return names, np.array(vals)
+
+
+def _rrm_oracle_with_theta(om_base, names, theta, lam):
+    import dataclasses
+
+    ipd = om_base.ipd_model
+    comps = {}
+    for lab, comp in ipd.comps.items():
+        kw = {k: theta[names.index(f"comps.{lab.value}.{k}")] for k in comp.init_params()}
+        comps[lab] = type(comp)(**kw)
+    om = O.OracleModel(lam * u.micron, model=dataclasses.replace(ipd, comps=comps))
+    for lab in ipd.comps:
+        om.comp_params[lab]["T_0"] = theta[names.index(f"T_0.{lab.value}")]
+        om.comp_params[lab]["delta"] = theta[names.index(f"delta.{lab.value}")]
+    om.shared_params["calibration"] = theta[names.index("calibration")]
+    return om
+
+
+def test_rrm_twin_values_and_gradient_match_oracle():
+    """RRM twin: values against the oracle, autograd against the oracle's central differences."""
+    from oracle.zodi_torch_twin import RRMTwin
+
+    lam = 25.0
+    om = O.OracleModel(lam * u.micron, name="rrm-experimental")
+    names, theta = _rrm_theta(om)
+    tw = RRMTwin(om, names)
+    v = random_dirs(300, 11)
+    obs = np.array([0.95, 0.3, 0.01])
+    got = tw.emission(torch.tensor(theta), v, obs, obs).numpy()
+    np.testing.assert_allclose(got, om.evaluate_vectors(v, obs, obs), rtol=1e-10, atol=0)
+    v = random_dirs(48, 12)
+    wts = np.random.default_rng(13).normal(size=v.shape[1])
+    th = torch.tensor(theta, requires_grad=True)
+    (tw.emission(th, v, obs, obs).sum(0) * torch.as_tensor(wts)).sum().backward()
+    g = th.grad.numpy()
+    picks = [names.index(n) for n in (
+        "comps.fan.gamma", "comps.fan.Z_0", "comps.fan.Q", "comps.fan.P", "comps.fan.i",
+        "comps.comet.gamma", "comps.comet.P", "comps.comet.amp", "comps.comet.Omega",
+        "comps.inner_narrow_band.G", "comps.inner_narrow_band.A",
+        "comps.inner_narrow_band.R_outer", "comps.outer_narrow_band.gamma",
+        "comps.broad_band.beta_bb", "comps.broad_band.sigma_bb", "comps.broad_band.z_0",
+        "comps.interstellar.amp", "comps.ring_rrm.A", "comps.feature_rrm.sigma_r",
+        "T_0.fan", "delta.broad_band", "calibration")]
+    for i in picks:
+        h = 1e-6 * max(abs(theta[i]), 1e-3)
+        tp, tm = theta.copy(), theta.copy()
+        tp[i] += h
+        tm[i] -= h
+        fp = (_rrm_oracle_with_theta(om, names, tp, lam).evaluate_vectors(v, obs, obs).sum(0)
+              * wts).sum()
+        fm = (_rrm_oracle_with_theta(om, names, tm, lam).evaluate_vectors(v, obs, obs).sum(0)
+              * wts).sum()
+        fd = (fp - fm) / (2 * h)
+        assert g[i] == pytest.approx(fd, rel=2e-6, abs=1e-9 * np.abs(g).max()), names[i]

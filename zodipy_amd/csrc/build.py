@@ -6,10 +6,12 @@ The library is built in-tree so it travels with the source snapshot.
 
 from __future__ import annotations
 
+import concurrent.futures
 import os
 import shutil
 import subprocess
 import sys
+import tempfile
 from pathlib import Path
 
 CSRC = Path(__file__).resolve().parent
@@ -52,10 +54,7 @@ def hipcc() -> str:
     raise RuntimeError("hipcc not found (ROCm is required to build the HIP library)")
 
 
-def build(verbose: bool = False) -> Path:
-    """Compile every HIP source into the in-tree shared library; return its path."""
-    tmp = LIB.with_suffix(".so.tmp")
-    cmd = [hipcc(), *FLAGS, f"-I{INCLUDE}", f"-I{CSRC}", *map(str, SOURCES), "-o", str(tmp)]
+def _run(cmd: list[str], verbose: bool) -> None:
     if verbose:
         print(" ".join(cmd), flush=True)
     proc = subprocess.run(cmd, capture_output=True, text=True)
@@ -64,7 +63,27 @@ def build(verbose: bool = False) -> Path:
         raise RuntimeError(f"hipcc failed with exit code {proc.returncode}")
     if verbose and proc.stderr.strip():
         sys.stderr.write(proc.stderr)
-    os.replace(tmp, LIB)
+
+
+def build(verbose: bool = False) -> Path:
+    """Compile every HIP source into the in-tree shared library; return its path.
+
+    Each source compiles to an object of its own, in parallel, in a temporary
+    directory; one link step then writes the library.
+    """
+    compile_flags = [f for f in FLAGS if f != "-shared"]
+    with tempfile.TemporaryDirectory(prefix="zodi_build_") as tmpdir:
+        objs = [Path(tmpdir) / (src.stem + ".o") for src in SOURCES]
+        cmds = [[hipcc(), *compile_flags, "-c", f"-I{INCLUDE}", f"-I{CSRC}", str(src), "-o",
+                 str(obj)] for src, obj in zip(SOURCES, objs)]
+        workers = max(1, min(len(cmds), os.cpu_count() or 1))
+        with concurrent.futures.ThreadPoolExecutor(workers) as pool:
+            for fut in [pool.submit(_run, c, verbose) for c in cmds]:
+                fut.result()
+        tmp = LIB.with_suffix(".so.tmp")
+        _run([hipcc(), f"--offload-arch={ARCH}", "-fPIC", "-shared", *map(str, objs), "-o",
+              str(tmp)], verbose)
+        os.replace(tmp, LIB)
     return LIB
 
 

@@ -502,6 +502,10 @@ AdjKernel pick_adjoint(const zodi_model* m, int chunk, int split, bool jac) {
       return pick_adj_chunk<LayoutDirbe>(chunk, split, jac);
     case kLayoutKelsall4:
       return pick_adj_chunk<LayoutKelsall4>(chunk, split, jac);
+    case kLayoutRRM:
+      return pick_adj_chunk<LayoutRRM>(chunk, split, jac);
+    case kLayoutRRMSplit:
+      return pick_adj_chunk<LayoutRRMSplit>(chunk, split, jac);
     default:
       return nullptr;
   }
@@ -518,6 +522,14 @@ int host_kind_nparams(int kind) {
       return kind_nparams<ZODI_RING>();
     case ZODI_FEATURE:
       return kind_nparams<ZODI_FEATURE>();
+    case ZODI_FAN:
+      return kind_nparams<ZODI_FAN>();
+    case ZODI_COMET:
+      return kind_nparams<ZODI_COMET>();
+    case ZODI_NARROW_BAND:
+      return kind_nparams<ZODI_NARROW_BAND>();
+    case ZODI_BROAD_BAND:
+      return kind_nparams<ZODI_BROAD_BAND>();
     case ZODI_INTERSTELLAR:
       return kind_nparams<ZODI_INTERSTELLAR>();
     case ZODI_RING_RRM:
@@ -685,8 +697,8 @@ AdjKernel pick_checked(const zodi_model* model_host, const AdjPlan& plan, bool j
   AdjKernel k = pick_adjoint(model_host, plan.chunk, plan.split, jac);
   if (k == nullptr) {
     st = fail(ZODI_ERR_MODEL,
-              "no adjoint for this component layout (supported: cloud, 3 bands[, ring, "
-              "feature] with the Kelsall integrand)");
+              "no adjoint for this component layout (supported: the registry layouts, i.e. "
+              "cloud, 3 bands[, ring, feature] with the Kelsall integrand, and the RRM model)");
     return nullptr;
   }
   st = validate_grad_layout(model_host);

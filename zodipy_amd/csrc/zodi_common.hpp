@@ -195,7 +195,20 @@ using LayoutRRM =
            Grp<ZODI_NARROW_BAND, ZODI_NARROW_BAND, ZODI_BROAD_BAND>, Grp<ZODI_INTERSTELLAR>,
            Grp<ZODI_RING_RRM>, Grp<ZODI_FEATURE_RRM>>;
 
-enum LayoutId { kLayoutGeneric = 0, kLayoutDirbe = 1, kLayoutKelsall4 = 2, kLayoutRRM = 3 };
+// The RRM model with one component per group, as parameter fits pack it (the
+// components' own temperature laws may then differ).
+using LayoutRRMSplit =
+    Layout<ZODI_INTEGRAND_RRM, Grp<ZODI_FAN>, Grp<ZODI_COMET>, Grp<ZODI_NARROW_BAND>,
+           Grp<ZODI_NARROW_BAND>, Grp<ZODI_BROAD_BAND>, Grp<ZODI_INTERSTELLAR>, Grp<ZODI_RING_RRM>,
+           Grp<ZODI_FEATURE_RRM>>;
+
+enum LayoutId {
+  kLayoutGeneric = 0,
+  kLayoutDirbe = 1,
+  kLayoutKelsall4 = 2,
+  kLayoutRRM = 3,
+  kLayoutRRMSplit = 4
+};
 
 inline bool layout_is(const zodi_model* m, int integrand, std::initializer_list<std::initializer_list<int>> groups) {
   if (m->integrand != integrand || m->n_groups != (int)groups.size()) return false;
@@ -224,6 +237,16 @@ inline LayoutId match_layout(const zodi_model* m) {
                  {ZODI_RING_RRM},
                  {ZODI_FEATURE_RRM}}))
     return kLayoutRRM;
+  if (layout_is(m, ZODI_INTEGRAND_RRM,
+                {{ZODI_FAN},
+                 {ZODI_COMET},
+                 {ZODI_NARROW_BAND},
+                 {ZODI_NARROW_BAND},
+                 {ZODI_BROAD_BAND},
+                 {ZODI_INTERSTELLAR},
+                 {ZODI_RING_RRM},
+                 {ZODI_FEATURE_RRM}}))
+    return kLayoutRRMSplit;
   return kLayoutGeneric;
 }
 

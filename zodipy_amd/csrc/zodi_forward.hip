@@ -122,6 +122,10 @@ struct ForwardOccupancy<LayoutRRM> {
   static constexpr int value = 4;
 };
 template <>
+struct ForwardOccupancy<LayoutRRMSplit> {
+  static constexpr int value = 4;
+};
+template <>
 struct ForwardOccupancy<LayoutKelsall4> {
   static constexpr int value = 4;
 };
@@ -340,6 +344,8 @@ FwdKernel pick_forward(const zodi_model* m, int block, int L, int variant) {
       return pick_layout<LayoutKelsall4>(block, L);
     case kLayoutRRM:
       return pick_layout<LayoutRRM>(block, L);
+    case kLayoutRRMSplit:
+      return pick_layout<LayoutRRMSplit>(block, L);
     default:
       return pick_generic(block, L);
   }

@@ -10,8 +10,8 @@
 // sigma_theta) are differentiated per degree. chain_geometry() turns dR, dZ,
 // dth into the partials with respect to x_0, y_0, z_0, i and Omega.
 //
-// The densities are those of zodipy/number_density.py:47-181 and :345-404;
-// |Z| is differentiated as sign(Z) (0 at Z = 0).
+// The densities are those of zodipy/number_density.py:47-404; |Z| is
+// differentiated as sign(Z) (0 at Z = 0).
 #pragma once
 
 #include "zodi_device.hpp"
@@ -26,6 +26,9 @@ constexpr int kind_nparams() {
   if constexpr (KIND == ZODI_CLOUD || KIND == ZODI_BAND) return 5;
   if constexpr (KIND == ZODI_RING) return 4;
   if constexpr (KIND == ZODI_FEATURE) return 6;
+  if constexpr (KIND == ZODI_FAN) return 5;
+  if constexpr (KIND == ZODI_COMET || KIND == ZODI_NARROW_BAND || KIND == ZODI_BROAD_BAND)
+    return 6;
   if constexpr (KIND == ZODI_INTERSTELLAR) return 1;
   if constexpr (KIND == ZODI_RING_RRM) return 5;
   if constexpr (KIND == ZODI_FEATURE_RRM) return 7;
@@ -169,6 +172,162 @@ __device__ __forceinline__ DensGrad<6> feature_grad(ParamPtr P, const CompFrame&
   return o;
 }
 
+// ------------------------------------------------------------ RRM components
+// The RRM densities (zodipy/number_density.py:184-342) depend on the ecliptic
+// latitude beta = asin(s), s = Z/R. At fixed R, d beta/dZ = 1/(R cos beta); at
+// fixed Z, d beta/dR = -s/(R cos beta). Mask edges (R_inner, and R_outer where it
+// only masks) move the quadrature's value in jumps, so their derivative is 0.
+// Points where the density is 0 (masked, or cos(beta)^Q = 0) get all-zero
+// partials.
+
+template <int NP>
+__device__ __forceinline__ DensGrad<NP> zero_grad() {
+  DensGrad<NP> o;
+  o.n = o.dR = o.dZ = o.dth = 0.0;
+#pragma unroll
+  for (int k = 0; k < NP; ++k) o.dp[k] = 0.0;
+  return o;
+}
+
+// Latitude exponent shared by the fan and the comet: u = |beta|^eps with
+// eps = 2 - |Z|/Z_0 for |Z| < Z_0 and 1 otherwise; returns u and fills
+// du/dbeta, du/deps (u log|beta|, 0 at beta = 0) and deps/dZ.
+struct LatTerm {
+  double beta, cb, u, du_db, du_deps, deps_dz, za;
+  bool inner;
+};
+
+__device__ __forceinline__ LatTerm lat_term(const CompFrame& f, double inv_Z0, double Z0) {
+  LatTerm t;
+  const double s = f.zc * f.inv_r;
+  t.beta = asin(s);
+  t.cb = cos(t.beta);
+  t.za = fabs(f.zc);
+  t.inner = t.za < Z0;
+  const double eps = t.inner ? 2.0 - t.za * inv_Z0 : 1.0;
+  const double L = fabs(t.beta);
+  t.u = pow_pos(L, eps);
+  const bool pos = L > 0.0;
+  t.du_db = pos ? eps * t.u * rcp_fast(L) * sign_of(t.beta) : 0.0;
+  t.du_deps = pos ? t.u * log_fast(L) : 0.0;
+  t.deps_dz = t.inner ? -sign_of(f.zc) * inv_Z0 : 0.0;
+  return t;
+}
+
+// Fan: cos(beta)^Q exp(-P sin(u)) R^-gamma for R <= R_outer. Partials in the
+// declaration order gamma, Z_0, Q, P, R_outer.
+__device__ __forceinline__ DensGrad<5> fan_grad(ParamPtr P, const CompFrame& f) {
+  const double Q = P[6], Pp = P[7], gamma = P[8], Z0 = P[9], Rout = P[10];
+  if (!(f.r <= Rout)) return zero_grad<5>();
+  const LatTerm t = lat_term(f, P[13], Z0);
+  const double su = sin(t.u), cu = cos(t.u);
+  const double fv = pow_pos(t.cb, Q) * exp_fast(-Pp * su);
+  DensGrad<5> o;
+  o.n = pow_radius(f.r, -gamma) * fv;
+  if (o.n == 0.0) return zero_grad<5>();
+  const double tanb = (f.zc * f.inv_r) * rcp_fast(t.cb);
+  const double dl_db = -Q * tanb - Pp * cu * t.du_db;  // d ln n / d beta
+  const double dl_deps = -Pp * cu * t.du_deps;
+  const double db_dz = f.inv_r * rcp_fast(t.cb);
+  o.dZ = o.n * (dl_db * db_dz + dl_deps * t.deps_dz);
+  o.dR = o.n * (-gamma * f.inv_r - dl_db * tanb * f.inv_r);
+  o.dth = 0.0;
+  o.dp[0] = -o.n * log_pos(f.r);
+  o.dp[1] = t.inner ? o.n * dl_deps * t.za * (P[13] * P[13]) : 0.0;
+  o.dp[2] = o.n * log_fast(t.cb);
+  o.dp[3] = -o.n * su;
+  o.dp[4] = 0.0;
+  return o;
+}
+
+// Comet: amp R^-gamma exp(-P sin(u)) for R_inner <= R <= R_outer. Partials in the
+// declaration order gamma, Z_0, P, amp, R_inner, R_outer.
+__device__ __forceinline__ DensGrad<6> comet_grad(ParamPtr P, const CompFrame& f) {
+  const double gamma = P[6], Z0 = P[7], Pp = P[8], amp = P[9], Rin = P[10], Rout = P[11];
+  if (!(f.r >= Rin && f.r <= Rout)) return zero_grad<6>();
+  const LatTerm t = lat_term(f, P[13], Z0);
+  const double su = sin(t.u), cu = cos(t.u);
+  const double rg = pow_radius(f.r, -gamma);
+  const double fv = exp_fast(-Pp * su);
+  DensGrad<6> o;
+  o.n = amp * rg * fv;
+  const double tanb = (f.zc * f.inv_r) * rcp_fast(t.cb);
+  const double dl_db = -Pp * cu * t.du_db;
+  const double dl_deps = -Pp * cu * t.du_deps;
+  const double db_dz = f.inv_r * rcp_fast(t.cb);
+  o.dZ = o.n * (dl_db * db_dz + dl_deps * t.deps_dz);
+  o.dR = o.n * (-gamma * f.inv_r - dl_db * tanb * f.inv_r);
+  o.dth = 0.0;
+  o.dp[0] = -o.n * log_pos(f.r);
+  o.dp[1] = t.inner ? o.n * dl_deps * t.za * (P[13] * P[13]) : 0.0;
+  o.dp[2] = -o.n * su;
+  o.dp[3] = rg * fv;
+  o.dp[4] = 0.0;
+  o.dp[5] = 0.0;
+  return o;
+}
+
+// Narrow band: A (R/R_outer)^-gamma exp(G (|beta_deg| - beta_nb)) for |beta_deg| <
+// beta_nb and R_inner <= R <= R_outer. Partials in the declaration order gamma, A,
+// G, R_inner, R_outer, beta_nb (degrees).
+__device__ __forceinline__ DensGrad<6> narrow_band_grad(ParamPtr P, const CompFrame& f) {
+  const double bnb = P[6], G = P[7], gamma = P[8], A = P[9], Rin = P[10], Rout = P[11];
+  if (!(f.r >= Rin && f.r <= Rout)) return zero_grad<6>();
+  const double s = f.zc * f.inv_r;
+  const double beta = asin(s);
+  const double babs = fabs(beta * kRad2Deg);
+  if (!(babs < bnb)) return zero_grad<6>();
+  const double fv = exp_fast(G * (babs - bnb));
+  const double rho = pow_radius(f.r * P[13], -gamma);
+  DensGrad<6> o;
+  o.n = A * rho * fv;
+  const double inv_cb = rcp_fast(cos(beta));
+  const double dl_db = G * sign_of(beta) * kRad2Deg;  // per radian
+  o.dZ = o.n * dl_db * f.inv_r * inv_cb;
+  o.dR = o.n * (-gamma * f.inv_r - dl_db * s * f.inv_r * inv_cb);
+  o.dth = 0.0;
+  o.dp[0] = -o.n * log_pos(f.r * P[13]);
+  o.dp[1] = rho * fv;
+  o.dp[2] = o.n * (babs - bnb);
+  o.dp[3] = 0.0;
+  o.dp[4] = o.n * gamma * P[13];
+  o.dp[5] = -G * o.n;
+  return o;
+}
+
+// Broad band: A (R/R_outer)^-gamma [exp(-a^2/2) + exp(-c^2/2)], a = (beta_deg -
+// beta_bb)/sigma_bb, c = (beta_deg + beta_bb)/sigma_bb, for R_inner <= R <= R_outer.
+// Partials in the declaration order gamma, A, R_inner, R_outer, beta_bb, sigma_bb.
+__device__ __forceinline__ DensGrad<6> broad_band_grad(ParamPtr P, const CompFrame& f) {
+  const double bbb = P[6], gamma = P[8], A = P[9], Rin = P[10], Rout = P[11];
+  const double inv_sig = P[13];
+  if (!(f.r >= Rin && f.r <= Rout)) return zero_grad<6>();
+  const double s = f.zc * f.inv_r;
+  const double beta = asin(s);
+  const double bdeg = beta * kRad2Deg;
+  const double a = (bdeg - bbb) * inv_sig;
+  const double c = (bdeg + bbb) * inv_sig;
+  const double ea = exp_fast(-0.5 * (a * a));
+  const double ec = exp_fast(-0.5 * (c * c));
+  const double fv = ea + ec;
+  const double rho = pow_radius(f.r * P[14], -gamma);
+  const double Ar = A * rho;
+  DensGrad<6> o;
+  o.n = Ar * fv;
+  const double inv_cb = rcp_fast(cos(beta));
+  const double dn_db = -Ar * (a * ea + c * ec) * inv_sig * kRad2Deg;  // per radian
+  o.dZ = dn_db * f.inv_r * inv_cb;
+  o.dR = -gamma * f.inv_r * o.n - dn_db * s * f.inv_r * inv_cb;
+  o.dth = 0.0;
+  o.dp[0] = -o.n * log_pos(f.r * P[14]);
+  o.dp[1] = rho * fv;
+  o.dp[2] = 0.0;
+  o.dp[3] = o.n * gamma * P[14];
+  o.dp[4] = Ar * (a * ea - c * ec) * inv_sig;
+  o.dp[5] = Ar * (a * a * ea + c * c * ec) * inv_sig;
+  return o;
+}
+
 // Value and parameter partials of a component whose kind is known at compile time.
 // `logR` = log |X| (heliocentric), reused where the component frame is the Sun's.
 template <int KIND>
@@ -184,6 +343,14 @@ __device__ __forceinline__ DensGrad<kind_nparams<KIND>()> density_grad(ParamPtr 
     return ring_grad(P, f);
   } else if constexpr (KIND == ZODI_FEATURE) {
     return feature_grad(P, f, earth_lon);
+  } else if constexpr (KIND == ZODI_FAN) {
+    return fan_grad(P, f);
+  } else if constexpr (KIND == ZODI_COMET) {
+    return comet_grad(P, f);
+  } else if constexpr (KIND == ZODI_NARROW_BAND) {
+    return narrow_band_grad(P, f);
+  } else if constexpr (KIND == ZODI_BROAD_BAND) {
+    return broad_band_grad(P, f);
   } else if constexpr (KIND == ZODI_INTERSTELLAR) {
     DensGrad<1> o;
     o.n = P[6];

@@ -38,8 +38,22 @@ import numpy as np
 import torch
 
 from zodipy_amd import _lib
-from zodipy_amd._lib import ADJ_CHI2, ADJ_VJP_COMPS, ADJ_VJP_SUM, OUT_COMPS, OUT_SUM, Launch
-from zodipy_amd.packing import PackedModel, pack_model, param_layout, param_vector
+from zodipy_amd._lib import (
+    ADJ_CHI2,
+    ADJ_VJP_COMPS,
+    ADJ_VJP_SUM,
+    OBS_PER_RAY,
+    OUT_COMPS,
+    OUT_SUM,
+    Launch,
+)
+from zodipy_amd.packing import (
+    PackedModel,
+    beyond_bits_for,
+    pack_model,
+    param_layout,
+    param_vector,
+)
 from zodipy_amd.runtime import DeviceModel, _f64_rows, _stream
 from zodipy_amd.zodiacal_light_model import Integrand
 
@@ -109,8 +123,11 @@ class ParameterSpace:
                 spec["comp"][lab]["T_0"] = float(theta[lay.spec_a[c]])
                 spec["comp"][lab]["delta"] = float(theta[lay.spec_b[c]])
             sh["calibration"] = float(theta[lay.shared[0]])
+        # Kelsall groups follow the fixed cutoffs and one shared temperature law;
+        # RRM components carry their own, so each keeps a group of its own.
         return pack_model(ipd, spec, self._b_nu_table, self._gl_x, self._gl_w,
-                          force_scatter=force_scatter)
+                          force_scatter=force_scatter,
+                          merge_groups=self._ipd.integrand == Integrand.KELSALL)
 
 
 class LineOfSightProblem:
@@ -139,11 +156,20 @@ class LineOfSightProblem:
                 msg = "several observers need seg_offsets or per_ray=True"
                 raise ValueError(msg)
             seg_offsets = np.array([0, self.n_rays], dtype=np.int64)
-        (self.seg, self.n_seg, self.obs, self.earth, self.beyond,
+        (self.seg, self.n_seg, self.obs, self.earth, beyond,
          self.mode) = base.prepare_observers(obs, earthpos, seg_offsets, per_ray)
         if self.seg is not None and int(self.seg[-1]) != self.n_rays:
             msg = f"seg_offsets[-1] = {int(self.seg[-1])} but there are {self.n_rays} rays"
             raise ValueError(msg)
+        # Observer distances to the Sun: the cutoff bits depend on the group radii
+        # of the block evaluated, and a parameter vector can regroup components
+        # (a temperature law that no longer matches its neighbour's).
+        o = self.obs
+        r_obs = torch.sqrt(o[0] * o[0] + o[1] * o[1] + o[2] * o[2])
+        if self.mode == OBS_PER_RAY:  # one call over all rays: only the farthest matters
+            r_obs = r_obs[~torch.isnan(r_obs)].max().reshape(1) if r_obs.numel() else r_obs
+        self._r_obs = r_obs.cpu().numpy()
+        self._beyond_cache = {self._group_key(base.packed.block): beyond}
         # Adjoint launch (None = library defaults); ``variant`` = components per pass.
         self.launch: Launch | None = None
         self.store_jacobian = bool(store_jacobian)
@@ -171,10 +197,29 @@ class LineOfSightProblem:
             self._theta_key = key
         return self._block
 
-    def _ptr_args(self):
+    @staticmethod
+    def _group_key(block: np.ndarray):
+        b = block.reshape(-1)[0]
+        ng = int(b["n_groups"])
+        return ng, b["group_r_in"][:ng].tobytes(), b["group_r_out"][:ng].tobytes()
+
+    def _beyond(self, host: np.ndarray) -> torch.Tensor:
+        """Cutoff bits of the observers for the groups of block ``host`` (cached)."""
+        key = self._group_key(host)
+        if key not in self._beyond_cache:
+            b = host.reshape(-1)[0]
+            ng = int(b["n_groups"])
+            bits = beyond_bits_for(self._r_obs, b["group_r_in"][:ng], b["group_r_out"][:ng],
+                                   any_over_all=self.mode == OBS_PER_RAY)
+            self._beyond_cache[key] = torch.from_numpy(bits.view(np.int32).copy()).to(
+                self.device)
+        return self._beyond_cache[key]
+
+    def _ptr_args(self, host: np.ndarray):
         return (self.u.data_ptr(), self.n_rays,
                 self.seg.data_ptr() if self.seg is not None else None, self.n_seg,
-                self.obs.data_ptr(), self.earth.data_ptr(), self.mode, self.beyond.data_ptr())
+                self.obs.data_ptr(), self.earth.data_ptr(), self.mode,
+                self._beyond(host).data_ptr())
 
     def forward(self, theta, return_comps: bool = False, blocks=None) -> torch.Tensor:
         """Integrated emission [MJy/sr]: (n_comps, n) or (n,).
@@ -185,7 +230,7 @@ class LineOfSightProblem:
         out_mode = OUT_COMPS if return_comps else OUT_SUM
         shape = (self.n_comps, self.n_rays) if return_comps else (self.n_rays,)
         out = torch.empty(shape, dtype=torch.float64, device=self.device)
-        st = self.lib.zodi_forward(_lib.host_ptr(host), dev.data_ptr(), *self._ptr_args(),
+        st = self.lib.zodi_forward(_lib.host_ptr(host), dev.data_ptr(), *self._ptr_args(host),
                                    out.data_ptr(), out_mode, None, _stream())
         _lib.check(st, "zodi_forward")
         return out
@@ -216,7 +261,7 @@ class LineOfSightProblem:
             return t.data_ptr() if t is not None else None
 
         st = self.lib.zodi_adjoint(
-            _lib.host_ptr(host), dev.data_ptr(), *self._ptr_args(), mode, ptr(weights),
+            _lib.host_ptr(host), dev.data_ptr(), *self._ptr_args(host), mode, ptr(weights),
             ptr(target), ptr(inv_var), float(inv_var_scalar), ptr(model_out), grad.data_ptr(),
             ws.data_ptr(), need, self.launch, _stream())
         _lib.check(st, "zodi_adjoint")
@@ -227,7 +272,7 @@ class LineOfSightProblem:
         host, dev, _ = self._model(theta)
         model = torch.empty(self.n_rays, dtype=torch.float64, device=self.device)
         jac = torch.empty((self.n_params, self.n_rays), dtype=torch.float64, device=self.device)
-        st = self.lib.zodi_jacobian(_lib.host_ptr(host), dev.data_ptr(), *self._ptr_args(),
+        st = self.lib.zodi_jacobian(_lib.host_ptr(host), dev.data_ptr(), *self._ptr_args(host),
                                     model.data_ptr(), jac.data_ptr(), self.n_rays, self.launch,
                                     _stream())
         _lib.check(st, "zodi_jacobian")

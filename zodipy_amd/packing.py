@@ -263,22 +263,35 @@ class PackedModel:
         obs = np.asarray(obs_xyz, dtype=np.float64).reshape(3, -1)
         x_0, y_0, z_0 = obs
         r_obs = np.sqrt(x_0**2 + y_0**2 + z_0**2)
-        bits = np.zeros(r_obs.shape, dtype=np.uint32)
-        for g in range(self.n_groups):
-            bits |= (r_obs > self.group_r_in[g]).astype(np.uint32) << np.uint32(2 * g)
-            bits |= (r_obs > self.group_r_out[g]).astype(np.uint32) << np.uint32(2 * g + 1)
-        if any_over_all:
-            return np.bitwise_or.reduce(bits, keepdims=True).astype(np.uint32)
-        return bits
+        return beyond_bits_for(r_obs, self.group_r_in, self.group_r_out, any_over_all)
+
+
+def beyond_bits_for(r_obs: np.ndarray, group_r_in, group_r_out, any_over_all: bool) -> np.ndarray:
+    """Cutoff masks from observer distances to the Sun and a block's group radii.
+
+    Bit 2g: r_obs > group_r_in[g]; bit 2g+1: r_obs > group_r_out[g] (NaN sets
+    neither); with ``any_over_all`` the OR over all observers.
+    """
+    r_obs = np.asarray(r_obs, dtype=np.float64)
+    bits = np.zeros(r_obs.shape, dtype=np.uint32)
+    for g in range(len(group_r_in)):
+        bits |= (r_obs > group_r_in[g]).astype(np.uint32) << np.uint32(2 * g)
+        bits |= (r_obs > group_r_out[g]).astype(np.uint32) << np.uint32(2 * g + 1)
+    if any_over_all:
+        return np.bitwise_or.reduce(bits, keepdims=True).astype(np.uint32)
+    return bits
 
 
 def pack_model(model: ZodiacalLightModel, spectral: dict, b_nu_table: np.ndarray,
-               gl_x: np.ndarray, gl_w: np.ndarray, force_scatter=()) -> PackedModel:
+               gl_x: np.ndarray, gl_w: np.ndarray, force_scatter=(),
+               merge_groups: bool = True) -> PackedModel:
     """Build the device block for ``model`` with effective ``spectral`` parameters.
 
     ``force_scatter``: labels of components whose scattering branch is kept even at
     albedo 0 (their albedo may change on the device; a zero albedo then adds an
-    exact zero).
+    exact zero). ``merge_groups=False`` gives every component a group of its own,
+    a structure that no parameter value can change (merged groups follow equal
+    temperature laws, which RRM parameter vectors can separate).
     """
     labels = list(model.comps.keys())
     n_comps = len(labels)
@@ -367,7 +380,8 @@ def pack_model(model: ZodiacalLightModel, spectral: dict, b_nu_table: np.ndarray
     for c, label in enumerate(labels):
         r_in, r_out = COMPONENT_CUTOFFS[label]
         key = (float(r_in), float(r_out), *temps_law[c])
-        if groups and keys[-1] == key and len(groups[-1]) < MAX_SLOTS_PER_GROUP:
+        if (merge_groups and groups and keys[-1] == key
+                and len(groups[-1]) < MAX_SLOTS_PER_GROUP):
             groups[-1].append(c)
         else:
             groups.append([c])
