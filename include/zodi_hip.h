@@ -304,8 +304,11 @@ enum {
  * sharing one ray's nodes), blocks_per_cu 0..64 (0 = 2), and variant the
  * components evaluated per pass over a group's nodes: 0 (default, 2), 1 or 2.
  * Supported layouts: the Kelsall registry models (cloud and three bands,
- * optionally followed by the ring and the feature); others return
- * ZODI_ERR_MODEL. The reduction order is fixed: equal inputs give equal bits.
+ * optionally followed by the ring and the feature) and the RRM model (its
+ * registry grouping, or one component per group); others return
+ * ZODI_ERR_MODEL. Mask edges of the RRM densities (R_inner, and R_outer where
+ * it only masks) have derivative 0. The reduction order is fixed: equal inputs
+ * give equal bits.
  */
 int64_t zodi_adjoint_workspace(const zodi_model* model_host, int64_t n_rays, int32_t mode,
                                const zodi_launch* launch);
