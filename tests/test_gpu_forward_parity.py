@@ -325,3 +325,35 @@ def test_host_mapped_evaluator_rejects_pageable_memory(gpu):
     out = torch.empty(int(seg[-1]), dtype=torch.float64).pin_memory()
     with pytest.raises(ValueError, match="pinned"):
         hm.run(torch.as_tensor(v).contiguous(), out)  # pageable pointings
+
+
+@pytest.mark.parametrize("x", [25 * u.micron, 1.25 * u.micron])
+def test_custom_layout_generic_kernel(gpu, cloud_ring_model, x):
+    """A registered model outside the registry layouts (cloud and ring only) runs
+    the generic kernel: parity with the oracle, scattering included at 1.25 um."""
+    from zodipy_amd.zodiacal_light_model import model_registry
+
+    v = random_dirs(1500, seed=31)
+    obs = np.array([0.62, -0.78, 0.05])
+    earth = np.array([0.60, -0.80, 0.0])
+    model = Model(x, name=cloud_ring_model)
+    got = model.evaluate_vectors(v, obspos=obs, earthpos=earth, return_comps=True).value
+    want = OracleModel(x, model=model_registry.get_model(cloud_ring_model)).evaluate_vectors(
+        v, obs, earth)
+    assert_parity(got, want)
+
+
+def test_rrm_one_component_per_group(gpu):
+    """The RRM model packed one component per group (as parameter fits pack it):
+    the specialised kernel for that structure matches the oracle."""
+    from zodipy_amd.fitting import LineOfSightProblem
+
+    v = random_dirs(1500, seed=32)
+    obs = np.array([0.62, -0.78, 0.05])
+    earth = np.array([0.60, -0.80, 0.0])
+    model = Model(60 * u.micron, name="rrm-experimental")
+    prob = LineOfSightProblem(model, v, obs, earth)
+    assert prob.space.pack(prob.space.theta0).n_groups == 8
+    got = prob.forward(prob.space.theta0, return_comps=True).cpu().numpy()
+    want = OracleModel(60 * u.micron, name="rrm-experimental").evaluate_vectors(v, obs, earth)
+    assert_parity(got, want)
