@@ -209,6 +209,80 @@ def test_feature_peaks_trailing_earth():
     assert abs(dth) < 1e-3
 
 
+def _latitude_point(comp, r, beta_rad, along=(1.0, 0.4, 0.0)):
+    """Point at distance r from X_0 at latitude beta above the symmetry plane."""
+    n = np.array([comp.sin_Omega_rad * comp.sin_i_rad, -comp.cos_Omega_rad * comp.sin_i_rad,
+                  comp.cos_i_rad])
+    a = (_in_plane_point(comp, 1.0, along) - comp.X_0.reshape(3))
+    return (comp.X_0.reshape(3) + r * (np.cos(beta_rad) * a + np.sin(beta_rad) * n)).reshape(3, 1)
+
+
+def test_rrm_density_closed_forms():
+    """Fan, comet, narrow and broad bands (zodipy/number_density.py:184-342) at points
+    where they reduce to closed forms: in the symmetry plane the latitude factors are
+    1 (fan, comet) or exp(-G beta_nb) (narrow band); at beta = 30 deg above Z_0 the
+    fan's exponent is 1 and sin(beta) = 1/2; the bands take latitudes in degrees;
+    every mask edge and the narrow band's latitude limit cut to exactly 0; the broad
+    band is even in beta with known values at beta = 0 and beta = beta_bb."""
+    rrm = model_registry.get_model("rrm-experimental")
+    L = ComponentLabel
+    fan, comet = rrm.comps[L.FAN], rrm.comps[L.COMET]
+    nb, bb = rrm.comps[L.INNER_NARROW_BAND], rrm.comps[L.BROAD_BAND]
+
+    def fan_n(X):
+        return O.fan_number_density(X, Q=fan.Q, P=fan.P, gamma=fan.gamma, Z_0=fan.Z_0,
+                                    R_outer=fan.R_outer, **_geom(fan))[0]
+
+    assert fan_n(_latitude_point(fan, 1.1, 0.0)) == pytest.approx(1.1**-fan.gamma, rel=1e-12)
+    # |Z| = 0.55 > Z_0 = 0.06: exponent 1, sin(pi/6) = 1/2, cos(pi/6) = sqrt(3)/2
+    want = 1.1**-fan.gamma * (np.sqrt(3) / 2) ** fan.Q * np.exp(-fan.P / 2)
+    assert fan_n(_latitude_point(fan, 1.1, np.pi / 6)) == pytest.approx(want, rel=1e-12)
+    assert fan_n(_latitude_point(fan, fan.R_outer * 1.001, 0.0)) == 0.0
+    assert fan_n(_latitude_point(fan, fan.R_outer * 0.999, 0.0)) > 0.0
+
+    def comet_n(X):
+        return O.comet_number_density(X, gamma=comet.gamma, Z_0=comet.Z_0, P=comet.P,
+                                      amp=comet.amp, R_inner=comet.R_inner,
+                                      R_outer=comet.R_outer, **_geom(comet))[0]
+
+    assert comet_n(_latitude_point(comet, 4.0, 0.0)) == pytest.approx(
+        comet.amp * 4.0**-comet.gamma, rel=1e-12)
+    assert comet_n(_latitude_point(comet, comet.R_inner * 0.999, 0.0)) == 0.0
+
+    def nb_n(X):
+        return O.narrow_band_number_density(X, beta_nb=nb.beta_nb, G=nb.G, gamma=nb.gamma, A=nb.A,
+                                            R_inner=nb.R_inner, R_outer=nb.R_outer,
+                                            **_geom(nb))[0]
+
+    r = 2.0
+    rho = (r / nb.R_outer) ** -nb.gamma
+    assert nb_n(_latitude_point(nb, r, 0.0)) == pytest.approx(
+        nb.A * rho * np.exp(-nb.G * nb.beta_nb), rel=1e-12)
+    half = np.radians(nb.beta_nb / 2)  # beta_nb is in degrees
+    assert nb_n(_latitude_point(nb, r, half)) == pytest.approx(
+        nb.A * rho * np.exp(-nb.G * nb.beta_nb / 2), rel=1e-10)
+    assert nb_n(_latitude_point(nb, r, -half)) == pytest.approx(nb_n(_latitude_point(nb, r, half)),
+                                                                rel=1e-12)
+    assert nb_n(_latitude_point(nb, r, np.radians(nb.beta_nb * 1.001))) == 0.0
+    assert nb_n(_latitude_point(nb, nb.R_outer * 1.001, 0.0)) == 0.0
+
+    def bb_n(X):
+        return O.broad_band_number_density(X, beta_bb=bb.beta_bb, sigma_bb=bb.sigma_bb,
+                                           gamma=bb.gamma, A=bb.A, R_inner=bb.R_inner,
+                                           R_outer=bb.R_outer, **_geom(bb))[0]
+
+    rho = (r / bb.R_outer) ** -bb.gamma
+    q = (bb.beta_bb / bb.sigma_bb) ** 2
+    assert bb_n(_latitude_point(bb, r, 0.0)) == pytest.approx(
+        2 * bb.A * rho * np.exp(-0.5 * q), rel=1e-12)
+    assert bb_n(_latitude_point(bb, r, np.radians(bb.beta_bb))) == pytest.approx(
+        bb.A * rho * (1 + np.exp(-2 * q)), rel=1e-10)
+    b = np.radians(13.0)
+    assert bb_n(_latitude_point(bb, r, b)) == pytest.approx(bb_n(_latitude_point(bb, r, -b)),
+                                                            rel=1e-12)
+    assert bb_n(_latitude_point(bb, bb.R_inner * 0.999, 0.0)) == 0.0
+
+
 def test_interstellar_is_constant():
     rrm = model_registry.get_model("rrm-experimental")
     ism = rrm.comps[ComponentLabel.INTERSTELLAR]
@@ -426,3 +500,30 @@ def test_rrm_twin_values_and_gradient_match_oracle():
               * wts).sum()
         fd = (fp - fm) / (2 * h)
         assert g[i] == pytest.approx(fd, rel=2e-6, abs=1e-9 * np.abs(g).max()), names[i]
+
+
+# --------------------------------------------------------------------- frames
+def _radec(v):
+    v = np.asarray(v, dtype=float)
+    return np.degrees(np.arctan2(v[1], v[0])) % 360.0, np.degrees(np.arcsin(v[2]))
+
+
+def test_frame_rotations_against_published_positions():
+    """The frame matrices (zodipy_amd/coords.py) against published J2000 positions:
+    the ecliptic pole at obliquity 84381.406 arcsec (IAU 2006) and the equinox on
+    the ICRS x axis, to the ~0.02 arcsec frame bias; the Galactic centre at ICRS
+    (266.40499, -28.93618) deg and the north galactic pole at (192.85948, 27.12825)
+    deg. A transposed matrix, a sign or a degree/radian slip moves these by degrees."""
+    from zodipy_amd import coords
+
+    M = coords.icrs_to_ecliptic_matrix()
+    eps = np.radians(84381.406 / 3600.0)
+    np.testing.assert_allclose(M @ [0.0, 0.0, 1.0], [0.0, np.sin(eps), np.cos(eps)], atol=3e-7)
+    np.testing.assert_allclose(M @ [1.0, 0.0, 0.0], [1.0, 0.0, 0.0], atol=3e-7)
+    G = coords.galactic_to_icrs_matrix()
+    ra, dec = _radec(G @ [1.0, 0.0, 0.0])
+    assert ra == pytest.approx(266.40499, abs=2e-5) and dec == pytest.approx(-28.93618, abs=2e-5)
+    ra, dec = _radec(G @ [0.0, 0.0, 1.0])
+    assert ra == pytest.approx(192.85948, abs=2e-5) and dec == pytest.approx(27.12825, abs=2e-5)
+    # the composed galactic -> ecliptic matrix is the product of the two
+    np.testing.assert_allclose(coords.to_ecliptic_matrix("galactic"), M @ G, atol=1e-15)
