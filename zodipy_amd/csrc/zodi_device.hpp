@@ -110,8 +110,12 @@ struct RayGeom {
   double r_obs;
 };
 
-__device__ __forceinline__ RayGeom ray_geometry(double ux, double uy, double uz, double ox,
-                                                double oy, double oz) {
+// Not inlined: the trigonometric routines' literal coefficients then live only
+// inside this once-per-ray call instead of in vector registers across the
+// kernels' node loops (where they were spilled to scratch); the call costs a
+// few instructions per ray.
+__device__ __attribute__((noinline)) RayGeom ray_geometry(double ux, double uy, double uz,
+                                                          double ox, double oy, double oz) {
   const double lon = atan2(uy, ux);
   const double lat = asin(uz);
   const double cos_lat = cos(lat);
