@@ -25,9 +25,12 @@ recomputes the integrand instead of storing node values:
 * ``LineOfSightProblem.jacobian(theta)`` -- the model values and the per-ray
   Jacobian dI_r/dtheta_p themselves, for Gauss-Newton or Fisher-matrix work.
 
-Supported models: the Kelsall registry layouts (cloud and three bands,
-optionally followed by the ring and the feature). Others raise RuntimeError
-from the library.
+Supported models: every registry model -- the Kelsall layouts (cloud and
+three bands, optionally followed by the ring and the feature) and the RRM model
+(packed one component per group, since a parameter vector may give its
+components different temperature laws). Other component combinations raise
+RuntimeError from the library. ``SpectralTableFit`` fits the spectral tables
+themselves on the device.
 """
 
 from __future__ import annotations
