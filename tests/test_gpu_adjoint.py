@@ -511,3 +511,31 @@ def test_separated_temperature_laws_match_oracle(gpu):
     th = torch.tensor(t, dtype=torch.float64, requires_grad=True)
     (tw.emission(th, v, OBS, EARTH).sum(0) * torch.as_tensor(w)).sum().backward()
     sensitivity_check(g, th.grad.numpy(), t, 1e-8, 1e-8, "regrouped vjp vs twin")
+
+
+def test_normal_equations_and_gauss_newton(setup):
+    """J^T W J and J^T W r against the Jacobian on the host; one undamped
+    Gauss-Newton step over the emissivities, in which the emission is linear at
+    25 um (albedo 0), recovers them exactly from noise-free data."""
+    model, _, names, theta, v, prob = setup
+    if model._x.value != 25.0:
+        pytest.skip("linear in the emissivities only without scattering")
+    I, J = prob.jacobian(theta)
+    rng = np.random.default_rng(30)
+    data = I * torch.as_tensor(1 + 0.01 * rng.normal(size=v.shape[1]), device="cuda")
+    iv = torch.as_tensor(rng.uniform(0.5, 2.0, size=v.shape[1]), device="cuda")
+    c2, g, H = prob.normal_equations(theta, data, iv, chunk_rays=100)
+    Jn, r, w = J.cpu().numpy(), (I - data).cpu().numpy(), iv.cpu().numpy()
+    np.testing.assert_allclose(H.cpu().numpy(), (Jn * w) @ Jn.T, rtol=1e-12,
+                               atol=1e-14 * np.abs((Jn * w) @ Jn.T).max())
+    np.testing.assert_allclose(g.cpu().numpy(), Jn @ (w * r), rtol=1e-12,
+                               atol=1e-14 * np.abs(Jn @ (w * r)).max())
+    assert float(c2) == pytest.approx(float((w * r * r).sum()), rel=1e-13)
+    emis = [i for i, n in enumerate(names) if n.startswith("emissivity.")]
+    truth = theta.copy()
+    truth[emis] *= 1 + 0.05 * rng.normal(size=len(emis))
+    data_t = prob.forward(truth)
+    new, c2_0 = prob.gauss_newton_step(theta, data_t, 1.0, free=emis)
+    np.testing.assert_allclose(new[emis], truth[emis], rtol=1e-9)
+    assert c2_0 > 0
+    assert np.array_equal(np.delete(new, emis), np.delete(theta, emis))

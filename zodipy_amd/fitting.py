@@ -281,6 +281,55 @@ class LineOfSightProblem:
         _lib.check(st, "zodi_jacobian")
         return model, jac
 
+    def normal_equations(self, theta, data, inv_var=None, chunk_rays: int = 1 << 22):
+        """(chi^2, J^T W r, J^T W J) at theta for a least-squares fit.
+
+        r = I - data, W = diag(inv_var) (None: 1; scalar or (n,)); J from
+        ``jacobian``. The two products are FP64 GEMMs (the library GEMM on the
+        GPU's matrix cores), accumulated over ray chunks so that only one
+        scaled copy of a chunk of J exists at a time. d chi^2/d theta = 2 J^T W r.
+        """
+        I, J = self.jacobian(theta)
+        d = self._rows(data, (self.n_rays,), "data")
+        if inv_var is None:
+            w = torch.ones_like(d)
+        elif isinstance(inv_var, torch.Tensor) and inv_var.ndim == 1:
+            w = self._rows(inv_var, (self.n_rays,), "inv_var")
+        else:
+            w = torch.full_like(d, float(inv_var))
+        r = I - d
+        wr = w * r
+        chi2 = torch.dot(r, wr)
+        g = torch.mv(J, wr)
+        H = torch.zeros((self.n_params, self.n_params), dtype=torch.float64, device=self.device)
+        for a in range(0, self.n_rays, chunk_rays):
+            Jc = J[:, a:a + chunk_rays]
+            H.addmm_(Jc * w[a:a + chunk_rays], Jc.t())
+        return chi2, g, H
+
+    def gauss_newton_step(self, theta, data, inv_var=None, damping: float = 0.0, free=None):
+        """One Levenberg-Marquardt step: theta + delta with
+        (H + damping diag(H)) delta = -J^T W r over the ``free`` entries
+        (boolean mask or indices; default: every entry with diag(H) > 0).
+
+        Returns (new theta as a NumPy array, chi^2 at theta).
+        """
+        th = self.space._check(theta)
+        chi2, g, H = self.normal_equations(th, data, inv_var)
+        diag = torch.diagonal(H)
+        if free is None:
+            mask = diag > 0
+        else:
+            mask = torch.zeros(self.n_params, dtype=torch.bool, device=self.device)
+            mask[torch.as_tensor(free, device=self.device)] = True
+            mask &= diag > 0
+        idx = torch.nonzero(mask).reshape(-1)
+        A = H[idx][:, idx] + damping * torch.diag(diag[idx])
+        delta = torch.linalg.solve(A, -g[idx])
+        out = th.copy()
+        out[idx.cpu().numpy()] += delta.cpu().numpy()
+        return out, float(chi2)
+
     def _rows(self, t, shape, name) -> torch.Tensor:
         t = torch.as_tensor(t, dtype=torch.float64, device=self.device)
         if tuple(t.shape) != shape:
