@@ -185,7 +185,9 @@ def grad_cost(dev, reps: int = 10) -> dict:
         chi^2 = sum (I - d)^2 / 0.01^2. The library's chi^2 call (the Jacobian
         pass, then J^T g) against the forward launch alone, both at theta0. The
         same chi^2 as a function of the spectral tables (SpectralTableFit: device
-        set-up of the effective values, chi^2, chain rule) is timed beside it.
+        set-up of the effective values, chi^2, chain rule) is timed beside it,
+        and the RRM model's chi^2 on the same rays (forward of its one-component-
+        per-group packing against the chi^2 call).
     (b) Planck-2018-style six-component model at 857 GHz, 30-frequency top-hat
         bandpass of +-10% (771.3-942.7 GHz, extrapolated past 857 GHz), 1000
         observers on a 1 AU orbit with seeded 5000-40000 pointings each: the
@@ -225,6 +227,19 @@ def grad_cost(dev, reps: int = 10) -> dict:
     }
     log(f"chi2 gradient (DIRBE 25 um, nside 128, {prob.n_params} params): fwd {t_fwd:.3f} ms, "
         f"fwd+grad {t_fg:.3f} ms, ratio {t_fg / t_fwd:.3f}")
+    rmodel = Model(25 * zu.micron, name="rrm-experimental", device=dev)
+    rprob = LineOfSightProblem(rmodel, v, obs, obs)
+    rth = rprob.space.theta0
+    rdata = rprob.forward(rth) * 1.01
+    rprob.chi2_and_grad(rth, rdata)
+    rt_fwd = _event_time(lambda: rprob.forward(rth), reps)
+    rt_fg = _event_time(lambda: rprob.chi2_and_grad(rth, rdata), reps)
+    out["chi2_rrm25_nside128"] = {
+        "n_rays": int(v.shape[1]), "n_params": rprob.n_params, "fwd_ms": round(rt_fwd, 4),
+        "fwd_grad_ms": round(rt_fg, 4), "ratio": round(rt_fg / rt_fwd, 4)}
+    log(f"chi2 gradient (RRM 25 um, nside 128, {rprob.n_params} params): fwd {rt_fwd:.3f} ms, "
+        f"fwd+grad {rt_fg:.3f} ms, ratio {rt_fg / rt_fwd:.3f}")
+    del rprob
     fit = SpectralTableFit(prob)
     vals = torch.as_tensor(fit.values0, device=dev)  # a device-resident fit's table vector
     fit.chi2_and_grad(vals, data, inv_var)
