@@ -301,7 +301,8 @@ def ingest_cost(dev, n_rays: int, reps: int = 5) -> dict:
 
     pix2vec: every pixel of a nside=2048 HEALPix map (50 331 648 pixels) in the
     galactic frame; lonlat: n_rays galactic longitude/latitude pairs (degrees);
-    track positions: n_rays observation times on a one-year hourly spline.
+    track positions: n_rays observation times on a one-year hourly spline;
+    the time-ordered pipeline: both, then the forward pass with per-ray observers.
     Algorithmic bytes: 24 written per element, plus 16 read for lonlat and 8 for
     the track times.
     """
@@ -344,6 +345,31 @@ def ingest_cost(dev, n_rays: int, reps: int = 5) -> dict:
             "times": "random" if times is mjd else "sorted",
             "GBs": round(32 * n_rays / (t / 1e3) / 1e9, 1),
             "frac_of_hbm": round(32 * n_rays / (t / 1e3) / 1e9 / PEAK_HBM_GBS, 3)}
+    # The time-ordered path end to end on the device: galactic lon/lat -> ecliptic
+    # vectors, sorted times -> positions on the track, then the forward with per-ray
+    # observers (one reference call over the span), against that forward alone.
+    from zodipy_amd import Model
+    from zodipy_amd import units as zu
+    from zodipy_amd.runtime import OUT_SUM
+
+    dm = Model(12 * zu.micron, device=dev).device_model
+
+    def pipeline():
+        vec = pointing.lonlat_to_vectors(lon, lat, frame="galactic", degrees=True)
+        pos = track.on_device(mjd_sorted, dev)
+        return dm.forward(vec, pos, pos, per_ray=True, out_mode=OUT_SUM)
+
+    vec0 = pointing.lonlat_to_vectors(lon, lat, frame="galactic", degrees=True)
+    pos0 = track.on_device(mjd_sorted, dev)
+    pipeline()
+    t_pipe = _event_time(pipeline, 3)
+    t_fwd = _event_time(lambda: dm.forward(vec0, pos0, pos0, per_ray=True, out_mode=OUT_SUM), 3)
+    out["tod_pipeline_12um"] = {
+        "n": n_rays, "ms": round(t_pipe, 3), "forward_only_ms": round(t_fwd, 3),
+        "overhead_share": round((t_pipe - t_fwd) / t_pipe, 4)}
+    log(f"time-ordered pipeline (lon/lat + track positions + forward, {n_rays} rays): "
+        f"{t_pipe:.2f} ms, forward alone {t_fwd:.2f} ms")
+    del vec0, pos0
     log(f"ingestion: pix2vec nside 2048 {out['pix2vec_nside2048_galactic']['ms']:.3f} ms "
         f"({out['pix2vec_nside2048_galactic']['GBs']} GB/s), lonlat {n_rays} "
         f"{out['lonlat_galactic']['ms']:.3f} ms ({out['lonlat_galactic']['GBs']} GB/s), "

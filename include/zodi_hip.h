@@ -19,6 +19,7 @@
  *   zodi_pix2vec,         zodipy/model.py:247-251 (the frame rotation and cartesian
  *   zodi_lonlat_to_vec    conversion of the pointings; HEALPix pixel centres)
  *   zodi_spline_positions zodipy/bodies.py:22-35 (time-ordered body positions)
+ *   zodi_max_radius       zodipy/line_of_sight.py:71-73 (the any-observer cutoff test)
  *   zodi_adjoint          no reference counterpart: vector-Jacobian product of the
  *                         forward map with respect to the model parameter vector
  *                         (the order of zodipy/zodiacal_light_model.py:38-55's
@@ -263,6 +264,16 @@ zodi_status zodi_lonlat_to_vec(const double* lon, const double* lat, int64_t n, 
  */
 zodi_status zodi_spline_positions(const double* records, int64_t n_intervals, const double* t,
                                   int64_t n, double* out, void* stream);
+
+/*
+ * zodi_max_radius — the largest distance to the Sun among per-ray observer
+ * positions xyz[3][n] (device memory), written to out[0] (-1 when n = 0 or every
+ * distance is NaN): the reference tests (r_obs > cutoff).any() over all
+ * observers of one call (zodipy/line_of_sight.py:71-73), so per-ray observers
+ * need this one number for their cutoff bits. out holds 1 + 1024 doubles of
+ * device memory (out[1..] receives block partials).
+ */
+zodi_status zodi_max_radius(const double* xyz, int64_t n, double* out, void* stream);
 
 /*
  * zodi_host_device_pointer — the device address of page-locked, mapped host

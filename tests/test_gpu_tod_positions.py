@@ -116,3 +116,28 @@ def test_per_ray_equals_per_segment_at_scale(gpu):
     tod = dm.forward(v, per_ray_obs, per_ray_obs, per_ray=True, out_mode=OUT_SUM)
     assert torch.equal(batched, tod)
     assert not torch.isnan(tod).any()
+
+
+@pytest.mark.parametrize("n", [0, 1, 1000, 3_000_001])
+def test_max_radius_kernel(gpu, n):
+    """zodi_max_radius against NumPy's max of sqrt(x^2 + y^2 + z^2), NaN skipped."""
+    from zodipy_amd import _lib
+    from zodipy_amd.runtime import _stream
+
+    rng = np.random.default_rng(n)
+    xyz = rng.normal(size=(3, n)) * rng.uniform(0.5, 2.0, size=n)
+    if n > 10:
+        xyz[:, 7] = np.nan
+        xyz[1, 9] = np.nan
+    t = torch.as_tensor(xyz, device="cuda").contiguous()
+    out = torch.empty(1025, dtype=torch.float64, device="cuda")
+    _lib.check(_lib.load().zodi_max_radius(t.data_ptr(), n, out.data_ptr(), _stream()),
+               "zodi_max_radius")
+    r = np.sqrt(xyz[0] ** 2 + xyz[1] ** 2 + xyz[2] ** 2)
+    want = np.nanmax(r) if n else -1.0
+    assert float(out[0]) == want
+    if n > 10:  # every distance NaN: -1, so no cutoff bit is set
+        t[:] = float("nan")
+        _lib.check(_lib.load().zodi_max_radius(t.data_ptr(), n, out.data_ptr(), _stream()),
+                   "zodi_max_radius")
+        assert float(out[0]) == -1.0

@@ -104,6 +104,9 @@ def test_host_validation_errors():
     assert lib.zodi_spline_positions(dst, 0, dst, 5, dst, None) == 1         # no interval
     assert lib.zodi_spline_positions(dst, 4, None, 5, dst, None) == 1
     assert lib.zodi_spline_positions(ct.c_void_p(0x3008), 4, dst, 5, dst, None) == 1  # align
+    assert lib.zodi_max_radius(dst, -1, dst, None) == 1
+    assert lib.zodi_max_radius(dst, 5, None, None) == 1
+    assert lib.zodi_max_radius(None, 5, dst, None) == 1
     out = ct.c_void_p()
     pageable = np.zeros(16)
     assert lib.zodi_host_device_pointer(pageable.ctypes.data, ct.byref(out)) == 1

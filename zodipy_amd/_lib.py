@@ -31,6 +31,7 @@ EXPORTED_SYMBOLS = (
     "zodi_lonlat_to_vec",
     "zodi_spline_positions",
     "zodi_host_device_pointer",
+    "zodi_max_radius",
     "zodi_adjoint_workspace",
     "zodi_adjoint",
     "zodi_jacobian",
@@ -76,6 +77,8 @@ def _bind(lib: ct.CDLL) -> None:
     lib.zodi_lonlat_to_vec.argtypes = [_p, _p, _i64, _i32, _p, _p, _p]
     lib.zodi_spline_positions.restype = ct.c_int32
     lib.zodi_spline_positions.argtypes = [_p, _i64, _p, _i64, _p, _p]
+    lib.zodi_max_radius.restype = ct.c_int32
+    lib.zodi_max_radius.argtypes = [_p, _i64, _p, _p]
     lib.zodi_host_device_pointer.restype = ct.c_int32
     lib.zodi_host_device_pointer.argtypes = [_p, ct.POINTER(ct.c_void_p)]
     lib.zodi_adjoint_workspace.restype = ct.c_int64

@@ -225,6 +225,55 @@ __global__ __launch_bounds__(256) void spline_kernel(const double* __restrict__ 
   }
 }
 
+// Largest observer distance to the Sun over per-ray positions [3][n]: the
+// reference tests (r_obs > cutoff).any() over all observers of one call
+// (zodipy/line_of_sight.py:71-73). Block b reduces r^2 = x^2 + y^2 + z^2 over a
+// grid-stride share (NaN compares false and is skipped) into part[b]; one block
+// then reduces the partials. sqrt is monotone and correctly rounded, so
+// sqrt(max r^2) is the maximum of the rounded distances.
+constexpr int kMaxRadiusBlocks = 1024;
+
+__global__ __launch_bounds__(256) void max_sq_radius_kernel(const double* __restrict__ xyz,
+                                                            int64_t n,
+                                                            double* __restrict__ part) {
+  __shared__ double red[256];
+  const int t = threadIdx.x;
+  double m = -1.0;
+  for (int64_t i = (int64_t)blockIdx.x * 256 + t; i < n; i += (int64_t)gridDim.x * 256) {
+    const double x = xyz[i], y = xyz[n + i], z = xyz[2 * n + i];
+    double r2;
+    {
+      // rounded term by term like the reference's x**2 + y**2 + z**2 (no FMA), so
+      // the cutoff comparison sees the reference's distance bit for bit
+#pragma clang fp contract(off)
+      r2 = x * x + y * y + z * z;
+    }
+    m = r2 > m ? r2 : m;
+  }
+  red[t] = m;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (t < w) red[t] = red[t + w] > red[t] ? red[t + w] : red[t];
+    __syncthreads();
+  }
+  if (t == 0) part[blockIdx.x] = red[0];
+}
+
+__global__ __launch_bounds__(256) void max_radius_finish(const double* __restrict__ part,
+                                                         int n_part, double* __restrict__ out) {
+  __shared__ double red[256];
+  const int t = threadIdx.x;
+  double m = -1.0;
+  for (int b = t; b < n_part; b += 256) m = part[b] > m ? part[b] : m;
+  red[t] = m;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (t < w) red[t] = red[t + w] > red[t] ? red[t + w] : red[t];
+    __syncthreads();
+  }
+  if (t == 0) out[0] = red[0] >= 0.0 ? sqrt(red[0]) : -1.0;
+}
+
 namespace {
 
 // Up to 256 blocks of 256 threads per CU, each thread looping over a few
@@ -295,6 +344,21 @@ zodi_status zodi_spline_positions(const double* records, int64_t n_intervals, co
   hipLaunchKernelGGL(spline_kernel, dim3((unsigned)elementwise_grid(n)), dim3(256), 0,
                      (hipStream_t)stream, records, n_intervals, t, n, out);
   return check_hip(hipGetLastError(), "spline launch");
+}
+
+zodi_status zodi_max_radius(const double* xyz, int64_t n, double* out, void* stream) {
+  if (n < 0) return fail(ZODI_ERR_ARG, "n < 0");
+  if (out == nullptr) return fail(ZODI_ERR_ARG, "out is null");
+  if (n > 0 && xyz == nullptr) return fail(ZODI_ERR_ARG, "xyz is null");
+  const int blocks =
+      (int)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, kMaxRadiusBlocks));
+  if (n > 0) {
+    hipLaunchKernelGGL(max_sq_radius_kernel, dim3(blocks), dim3(256), 0, (hipStream_t)stream,
+                       xyz, n, out + 1);
+  }
+  hipLaunchKernelGGL(max_radius_finish, dim3(1), dim3(256), 0, (hipStream_t)stream,
+                     (const double*)(out + 1), n > 0 ? blocks : 0, out);
+  return check_hip(hipGetLastError(), "max radius launch");
 }
 
 }  // extern "C"

@@ -80,9 +80,11 @@ class DeviceModel:
             msg = f"earth positions {tuple(earth.shape)} do not match observers {tuple(obs.shape)}"
             raise ValueError(msg)
         # r = sqrt(x^2 + y^2 + z^2) as zodipy/line_of_sight.py:71 forms it; NaN distances
-        # compare false, as in (r_obs > cutoff).any()
-        r = torch.sqrt(obs[0] * obs[0] + obs[1] * obs[1] + obs[2] * obs[2])
-        r_max = float(torch.nan_to_num(r, nan=-1.0).max()) if r.numel() else -1.0
+        # compare false, as in (r_obs > cutoff).any(). One fused reduction kernel.
+        red = torch.empty(1025, dtype=torch.float64, device=self.device)
+        _lib.check(_lib.load().zodi_max_radius(obs.data_ptr(), obs.shape[1], red.data_ptr(),
+                                               _stream()), "zodi_max_radius")
+        r_max = float(red[0])
         beyond = self.packed.beyond_bits(np.array([[r_max], [0.0], [0.0]]), any_over_all=True)
         beyond_t = torch.from_numpy(beyond.view(np.int32).copy()).to(self.device)
         return None, 0, obs, earth, beyond_t, OBS_PER_RAY
