@@ -43,7 +43,7 @@ def random_dirs(n, seed):
     return np.stack([s * np.cos(phi), s * np.sin(phi), z])
 
 
-def test_config1_dirbe_25um_nside16(gpu):
+def test_dirbe_25um_nside16_one_observer(gpu):
     """DIRBE, 25 um, nside=16 pixel centres, observer = Earth = (1, 0, 0) AU."""
     v = healpix.pix2vec(16)
     obs = np.array([1.0, 0.0, 0.0])

@@ -29,7 +29,7 @@ from zodipy_amd.spectral import (
 H, KB, C, MJY = 6.62607015e-34, 1.380649e-23, 299792458.0, 1e-20
 
 
-def config2_bandpass():
+def gaussian_bandpass_25um():
     x = np.linspace(17.5, 32.5, 50) * u.micron
     return x, np.exp(-0.5 * ((x.value - 25.0) / 2.5) ** 2)
 
@@ -37,7 +37,7 @@ def config2_bandpass():
 CASES = {
     "dirbe_25um": dict(x=25 * u.micron),
     "dirbe_18.5um": dict(x=18.5 * u.micron),
-    "dirbe_bandpass_um": dict(x=config2_bandpass()[0], weights=config2_bandpass()[1]),
+    "dirbe_bandpass_um": dict(x=gaussian_bandpass_25um()[0], weights=gaussian_bandpass_25um()[1]),
     "dirbe_bandpass_ghz": dict(x=np.linspace(11000, 13000, 20) * u.GHz, weights=np.ones(20)),
     "planck18_857": dict(x=857 * u.GHz, name="planck18"),
     "planck18_tophat": dict(x=np.linspace(771.3, 942.7, 30) * u.GHz, weights=np.ones(30),
@@ -98,7 +98,7 @@ def test_closed_forms_of_the_weights():
     assert mid["C1"].target == SPEC_C1
     np.testing.assert_array_equal(mid["C1"].weights, np.eye(10)[4])
     # a normalised bandpass in the model's own unit: the weights sum to one
-    x, w = config2_bandpass()
+    x, w = gaussian_bandpass_25um()
     bp = {t.name: t for t in _tables(Model(x, weights=w))}
     assert abs(bp["emissivity.feature"].weights.sum() - 1.0) <= 1e-14
     assert np.all(bp["emissivity.feature"].weights >= 0)
