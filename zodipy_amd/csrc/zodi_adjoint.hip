@@ -77,9 +77,13 @@ __device__ __forceinline__ double planck_lookup_slope(double T, const double* __
   return T == T ? B : T;
 }
 
-template <int NP>
+// Register accumulators of one component: the geometry moments (geo_moments), the
+// kind's parameters (NP) and the two spectral parameters.
+template <int KIND>
 struct CompAcc {
-  double v[5 + NP + 2];  // geometry (5), kind parameters (NP), two spectral
+  static constexpr int G = geo_moments<is_feature(KIND)>();
+  static constexpr int NP = kind_nparams<KIND>();
+  double v[G + NP + 2];
 };
 
 // Components J (consecutive, from S) of group g, whose first component is `first`.
@@ -135,7 +139,7 @@ __device__ __forceinline__ void adjoint_group(Grp<Ks...>, std::integer_sequence<
                                          M->shared[ZODI_S_C3] != 0.0);
   const double phi_const = M->shared[ZODI_S_PHASE_NORM] * (M->shared[ZODI_S_C1] + 1.0);
 
-  std::tuple<CompAcc<kind_nparams<kinds[J]>()>...> acc{};
+  std::tuple<CompAcc<kinds[J]>...> acc{};
   double Iv[N];  // JAC: node sums of each component's integrand, as the forward forms them
   ((Iv[J - S] = 0.0), ...);
 
@@ -162,16 +166,16 @@ __device__ __forceinline__ void adjoint_group(Grp<Ks...>, std::integer_sequence<
       pg = phase_grad(y.ux, y.uy, y.uz, nd.X0, nd.X1, nd.X2, nd.inv_R, Mk->shared);
     }
     const double inv_R2 = nd.inv_R * nd.inv_R;
+    double sw = 0.0;  // Kelsall: sum over the chunk of w n (1 - a) eps, for T_0 and delta
     (
         [&] {
           constexpr int KIND = kinds[J];
           constexpr int NP = kind_nparams<KIND>();
+          constexpr int G = geo_moments<is_feature(KIND)>();
           const ParamPtr P = Mk->comp[first + J];
           const uint32_t flags = Mk->comp_flags[first + J];
           const CompFrame f = comp_frame(P, flags, nd.X0, nd.X1, nd.X2, nd.R, nd.inv_R);
           const DensGrad<NP> dg = density_grad<KIND>(P, flags, f, elon[J - S], logR);
-          double geo[5];
-          chain_geometry<is_feature(KIND)>(P, f, dg.dR, dg.dZ, dg.dth, et[J - S], geo);
           auto& A = std::get<J - S>(acc).v;
           const double wn = wk * W[J - S];
           if constexpr (INTEGRAND == ZODI_INTEGRAND_KELSALL) {
@@ -185,39 +189,44 @@ __device__ __forceinline__ void adjoint_group(Grp<Ks...>, std::integer_sequence<
               Iv[J - S] += (flags & ZODI_CF_SCATTER) ? wk * (e * dg.n) : (wk * nd.B) * dg.n;
             }
             const double we = wn * e;
+            geo_accumulate<is_feature(KIND)>(A, we, f, dg.dR, dg.dZ, dg.dth);
 #pragma unroll
-            for (int q = 0; q < 5; ++q) A[q] += we * geo[q];
-#pragma unroll
-            for (int q = 0; q < NP; ++q) A[5 + q] += we * dg.dp[q];
+            for (int q = 0; q < NP; ++q) A[G + q] += we * dg.dp[q];
             const double wnn = wn * dg.n;
-            A[5 + NP] += wnn * (oma * nd.B);                  // emissivity
-            A[5 + NP + 1] += wnn * (scat_unit - eps * nd.B);  // albedo
+            A[G + NP] += wnn * (oma * nd.B);                  // emissivity
+            A[G + NP + 1] += wnn * (scat_unit - eps * nd.B);  // albedo
             // (1 - a) eps, packed with the same rounding as oma * eps
-            const double wt = wnn * P[ZODI_P_SCALE] * dBdT;
-            sh[0] += wt * dT_dT0;
-            sh[1] += wt * dT_ddelta;
-            const double was = wnn * alb * inv_R2;
-            sh[2] += was * pg.phi;
-            const double wasf = was * solar;
-            sh[3] += wasf * pg.d1;
-            sh[4] += wasf * pg.d2;
-            sh[5] += wasf * pg.d3;
+            sw += wnn * P[ZODI_P_SCALE];
+            // F, C1..C3 enter through a Phi / R^2, which is 0 when a == 0
+            if (flags & ZODI_CF_SCATTER) {
+              const double was = wnn * alb * inv_R2;
+              sh[2] += was * pg.phi;
+              const double wasf = was * solar;
+              sh[3] += wasf * pg.d1;
+              sh[4] += wasf * pg.d2;
+              sh[5] += wasf * pg.d3;
+            }
           } else {
             // f = calibration B(T_c) n   (zodipy/brightness.py:59-83)
             if constexpr (JAC) Iv[J - S] += (wk * nd.B) * dg.n;
             const double wc = wn * cx.calibration;
             const double wcb = wc * nd.B;
+            geo_accumulate<is_feature(KIND)>(A, wcb, f, dg.dR, dg.dZ, dg.dth);
 #pragma unroll
-            for (int q = 0; q < 5; ++q) A[q] += wcb * geo[q];
-#pragma unroll
-            for (int q = 0; q < NP; ++q) A[5 + q] += wcb * dg.dp[q];
+            for (int q = 0; q < NP; ++q) A[G + q] += wcb * dg.dp[q];
             const double wt = wc * dg.n * dBdT;
-            A[5 + NP] += wt * dT_dT0;         // T_0 of this component
-            A[5 + NP + 1] += wt * dT_ddelta;  // delta of this component
+            A[G + NP] += wt * dT_dT0;         // T_0 of this component
+            A[G + NP + 1] += wt * dT_ddelta;  // delta of this component
             sh[0] += wn * nd.B * dg.n;        // calibration
           }
         }(),
         ...);
+    if constexpr (INTEGRAND == ZODI_INTEGRAND_KELSALL) {
+      // T = T_0 R^-delta is shared by the chunk's components
+      const double wt = sw * dBdT;
+      sh[0] += wt * dT_dT0;
+      sh[1] += wt * dT_ddelta;
+    }
   }
 
   if constexpr (JAC) {
@@ -225,18 +234,23 @@ __device__ __forceinline__ void adjoint_group(Grp<Ks...>, std::integer_sequence<
     (
         [&] {
           constexpr int NP = kind_nparams<kinds[J]>();
+          constexpr int G = geo_moments<is_feature(kinds[J])>();
           const auto& A = std::get<J - S>(acc).v;
           const int c = first + J;
           total = __dadd_rn(total, finish<INTEGRAND>(M->comp[c], M->comp_flags[c], Iv[J - S], h,
                                                      cx.calibration));
+          double geo[5];
+          geo_finish<is_feature(kinds[J])>(M->comp[c], A, et[J - S], geo);
           if (y.active) {
             double* col = a.jac + y.r;
             const int64_t ld = a.ld_jac;
             const int off = M->grad_comp[c];
 #pragma unroll
-            for (int q = 0; q < 5 + NP; ++q) col[(off + q) * ld] = A[q];
-            col[M->grad_spec_a[c] * ld] = A[5 + NP];
-            col[M->grad_spec_b[c] * ld] = A[5 + NP + 1];
+            for (int q = 0; q < 5; ++q) col[(off + q) * ld] = geo[q];
+#pragma unroll
+            for (int q = 0; q < NP; ++q) col[(off + 5 + q) * ld] = A[G + q];
+            col[M->grad_spec_a[c] * ld] = A[G + NP];
+            col[M->grad_spec_b[c] * ld] = A[G + NP + 1];
           }
         }(),
         ...);
@@ -245,16 +259,23 @@ __device__ __forceinline__ void adjoint_group(Grp<Ks...>, std::integer_sequence<
     (
         [&] {
           constexpr int NP = kind_nparams<kinds[J]>();
+          constexpr int G = geo_moments<is_feature(kinds[J])>();
           auto& A = std::get<J - S>(acc).v;
+          const int c = first + J;
+          double geo[5];
+          geo_finish<is_feature(kinds[J])>(M->comp[c], A, et[J - S], geo);
 #pragma unroll
-          for (int q = 0; q < 5 + NP + 2; ++q) A[q] = wave_sum(A[q]);
+          for (int q = 0; q < 5; ++q) geo[q] = wave_sum(geo[q]);
+#pragma unroll
+          for (int q = G; q < G + NP + 2; ++q) A[q] = wave_sum(A[q]);
           if (lane == 0) {
-            const int c = first + J;
             const int off = M->grad_comp[c];
 #pragma unroll
-            for (int q = 0; q < 5 + NP; ++q) row[off + q] += A[q];
-            row[M->grad_spec_a[c]] += A[5 + NP];
-            row[M->grad_spec_b[c]] += A[5 + NP + 1];
+            for (int q = 0; q < 5; ++q) row[off + q] += geo[q];
+#pragma unroll
+            for (int q = 0; q < NP; ++q) row[off + 5 + q] += A[G + q];
+            row[M->grad_spec_a[c]] += A[G + NP];
+            row[M->grad_spec_b[c]] += A[G + NP + 1];
           }
         }(),
         ...);

@@ -7,8 +7,8 @@
 // and the partials with respect to the component's own parameters, in the
 // declaration order of its parameter dataclass after (x_0, y_0, z_0, i, Omega).
 // Angles given in degrees in the parameter dictionary (delta_zeta, theta,
-// sigma_theta) are differentiated per degree. chain_geometry() turns dR, dZ,
-// dth into the partials with respect to x_0, y_0, z_0, i and Omega.
+// sigma_theta) are differentiated per degree. geo_accumulate() and geo_finish()
+// turn dR, dZ, dth into the partials with respect to x_0, y_0, z_0, i and Omega.
 //
 // The densities are those of zodipy/number_density.py:47-404; |Z| is
 // differentiated as sign(Z) (0 at Z = 0).
@@ -73,16 +73,17 @@ __device__ __forceinline__ DensGrad<5> cloud_grad(ParamPtr P, uint32_t flags, co
   const double dg_dzeta = inner ? zeta * (2.0 * inv_2mu) : 1.0;
   // gamma g^(gamma-1) written as gamma G / g; its limit at g = 0 is 0 for gamma > 1/2.
   const double gG = g > 0.0 ? gamma * G * rcp_fast(g) : 0.0;
-  const double dn_dzeta = -o.n * beta * gG * dg_dzeta;
-  o.dR = -alpha * o.n * f.inv_r - dn_dzeta * zeta * f.inv_r;
+  const double nbg = o.n * beta * gG;  // -dn/dg
+  const double dn_dzeta = -nbg * dg_dzeta;
+  o.dR = -(alpha * o.n + dn_dzeta * zeta) * f.inv_r;
   o.dZ = dn_dzeta * sign_of(f.zc) * f.inv_r;
   o.dth = 0.0;
   o.dp[0] = E;
   o.dp[1] = -o.n * logr;
   o.dp[2] = -o.n * G;
-  o.dp[3] = g > 0.0 ? -o.n * beta * G * logg : 0.0;
+  o.dp[3] = g > 0.0 ? -(o.n * beta) * G * logg : 0.0;
   const double dg_dmu = inner ? -(zeta * zeta) * (2.0 * inv_2mu * inv_2mu) : -0.5;
-  o.dp[4] = -o.n * beta * gG * dg_dmu;
+  o.dp[4] = -nbg * dg_dmu;
   return o;
 }
 
@@ -104,21 +105,24 @@ __device__ __forceinline__ DensGrad<5> band_grad(ParamPtr P, uint32_t flags, con
   const double ex = exp_fast(-x);
   const double t4 = 1.0 - ex;
   DensGrad<5> o;
-  const double t124 = t1 * t2 * t4;
+  const double t24 = t2 * t4;
+  const double t124 = t1 * t24;
   o.n = t124 * t3;
   // d(zd^p)/dzd = p zd^(p-1) (taken as 0 at zd = 0 for p > 1)
   const double dzdp = (flags & ZODI_CF_P_IS_4) ? 4.0 * (zd2 * zd)
                                                : (zd > 0.0 ? p * zdp * rcp_fast(zd) : 0.0);
-  const double dn_dzd = t1 * t4 * t2 * (-6.0 * zd5 * t3 + dzdp * inv_v);
-  const double t123_ex = t1 * t2 * t3 * ex;
-  o.dR = -o.n * f.inv_r - dn_dzd * zd * f.inv_r + t123_ex * 20.0 * x * f.inv_r;
-  o.dZ = dn_dzd * sign_of(f.zc) * f.inv_r * inv_dzeta;
+  const double dn_dzd = t124 * (-6.0 * zd5 * t3 + dzdp * inv_v);
+  // R' d/dR' of the radial cut-off factor: 20 x exp(-x) / (1 - exp(-x)) times n
+  const double rdr = (t1 * t2) * t3 * (20.0 * x) * ex;
+  o.dR = (rdr - o.n - dn_dzd * zd) * f.inv_r;
+  o.dZ = dn_dzd * sign_of(f.zc) * (f.inv_r * inv_dzeta);
   o.dth = 0.0;
-  o.dp[0] = 3.0 * t2 * t3 * t4 * f.inv_r;
-  o.dp[1] = -dn_dzd * zd * inv_dzeta * kDegToRad;
-  o.dp[2] = -t124 * zdp * inv_v * inv_v;
-  o.dp[3] = zd > 0.0 ? t124 * zdp * logzd * inv_v : 0.0;
-  o.dp[4] = -t123_ex * 20.0 * x * inv_dr;
+  o.dp[0] = 3.0 * (t24 * t3) * f.inv_r;
+  o.dp[1] = -dn_dzd * zd * (inv_dzeta * kDegToRad);
+  const double nzv = t124 * zdp * inv_v;  // dn/dv = -nzv / v, dn/dp = nzv log zd
+  o.dp[2] = -nzv * inv_v;
+  o.dp[3] = zd > 0.0 ? nzv * logzd : 0.0;
+  o.dp[4] = -rdr * inv_dr;
   (void)dzeta;
   (void)v;
   (void)dr;
@@ -398,23 +402,58 @@ __device__ __forceinline__ EarthTerm earth_term(ParamPtr P, double ex, double ey
 // Partials with respect to x_0, y_0, z_0, i [deg], Omega [deg] from dR, dZ, dth.
 // X' = X - X_0; Z = x' sin(Omega) sin(i) - y' cos(Omega) sin(i) + z' cos(i); for the
 // feature theta_comp = atan2(y', x') - atan2(E'_y, E'_x) with E' = X_earth - X_0.
+// Per point,
+//   d/dx_0 = -(dR x'/R' + dZ n_x) [+ dth y'/rho^2 - dth E'_y/|E'_xy|^2],
+//   d/dy_0 = -(dR y'/R' - dZ n_y) [+ dth E'_x/|E'_xy|^2 - dth x'/rho^2],
+//   d/dz_0 = -(dR z'/R' + dZ n_z),
+//   d/di   = dZ (x' sin(Omega) cos(i) - y' cos(Omega) cos(i) - z' sin(i)) pi/180,
+//   d/dOmega = dZ (x' cos(Omega) sin(i) + y' sin(Omega) sin(i)) pi/180,
+// with (n_x, n_y, n_z) the plane normal and rho^2 = x'^2 + y'^2. The component's
+// constants and the ray's Earth terms factor out of the node sums, so the node
+// loop accumulates the weighted moments (geo_accumulate) and geo_finish combines
+// them once per ray.
 template <bool FEATURE>
-__device__ __forceinline__ void chain_geometry(ParamPtr P, const CompFrame& f, double dR,
-                                               double dZ, double dth, const EarthTerm& et,
-                                               double out[5]) {
+constexpr int geo_moments() {
+  return FEATURE ? 10 : 7;
+}
+
+// Moments: sum w dR x'/R', y', z' terms (0-2); sum w dZ x', y', z' (3-5); sum w dZ (6);
+// feature: sum w dth y'/rho^2, x'/rho^2 (7, 8) and sum w dth (9).
+template <bool FEATURE>
+__device__ __forceinline__ void geo_accumulate(double* A, double w, const CompFrame& f, double dR,
+                                               double dZ, double dth) {
+  const double wr = w * (dR * f.inv_r);
+  const double wz = w * dZ;
+  A[0] += wr * f.x;
+  A[1] += wr * f.y;
+  A[2] += wr * f.z;
+  A[3] += wz * f.x;
+  A[4] += wz * f.y;
+  A[5] += wz * f.z;
+  A[6] += wz;
+  if constexpr (FEATURE) {
+    const double wt = w * dth;
+    const double wtr = wt * rcp_fast(f.x * f.x + f.y * f.y);
+    A[7] += wtr * f.y;
+    A[8] += wtr * f.x;
+    A[9] += wt;
+  }
+}
+
+template <bool FEATURE>
+__device__ __forceinline__ void geo_finish(ParamPtr P, const double* A, const EarthTerm& et,
+                                           double out[5]) {
   const double si = P[ZODI_P_SIN_I], ci = P[ZODI_P_COS_I];
   const double so = P[ZODI_P_SIN_OMEGA], co = P[ZODI_P_COS_OMEGA];
-  const double dRr = dR * f.inv_r;
-  out[0] = -(dRr * f.x + dZ * P[ZODI_P_NX]);
-  out[1] = -(dRr * f.y - dZ * P[ZODI_P_NY]);
-  out[2] = -(dRr * f.z + dZ * P[ZODI_P_NZ]);
+  out[0] = -(A[0] + A[6] * P[ZODI_P_NX]);
+  out[1] = -(A[1] - A[6] * P[ZODI_P_NY]);
+  out[2] = -(A[2] + A[6] * P[ZODI_P_NZ]);
   if constexpr (FEATURE) {
-    const double dth_rho2 = dth * rcp_fast(f.x * f.x + f.y * f.y);
-    out[0] += dth_rho2 * f.y - dth * et.gx;
-    out[1] += dth * et.gy - dth_rho2 * f.x;
+    out[0] += A[7] - A[9] * et.gx;
+    out[1] += A[9] * et.gy - A[8];
   }
-  out[3] = dZ * (f.x * so * ci - f.y * co * ci - f.z * si) * kDegToRad;
-  out[4] = dZ * (f.x * co * si + f.y * so * si) * kDegToRad;
+  out[3] = (A[3] * (so * ci) - A[4] * (co * ci) - A[5] * si) * kDegToRad;
+  out[4] = (A[3] * (co * si) + A[4] * (so * si)) * kDegToRad;
 }
 
 // Scattering phase function and its partials with respect to C1, C2, C3
