@@ -237,6 +237,35 @@ def test_chi2_paths_agree_on_segments_and_per_ray(gpu):
                           "jacobian vs recompute chi2 gradient")
 
 
+@pytest.mark.parametrize("n_rays", [786_432, 786_431], ids=["even", "odd"])
+def test_chi2_jacobian_ray_pairs(gpu, n_rays):
+    """The chi^2 reduction over a stored Jacobian at sizes where its blocks take
+    768 rays (512 as pairs, the rest one by one): against the recompute path, and
+    (even n) the same bits whether J's rows allow 16-byte loads or not."""
+    v = healpix.pix2vec(256)[:, :n_rays]
+    prob = LineOfSightProblem(Model(25 * u.micron), v, OBS, EARTH)
+    theta = prob.space.theta0
+    rng = np.random.default_rng(18)
+    I0 = prob.forward(theta)
+    data = I0 * torch.as_tensor(1 + 0.01 * rng.normal(size=n_rays), device="cuda")
+    iv = torch.as_tensor(rng.uniform(0.5, 2.0, size=n_rays), device="cuda")
+    c_j, g_j = prob.chi2_and_grad(theta, data, iv)
+    if n_rays % 2 == 0:
+        # the same problem with the workspace shifted by one double (8-byte aligned rows)
+        ws = prob._workspace
+        shifted = torch.empty(ws.numel() + 1, dtype=torch.float64, device="cuda")[1:]
+        assert shifted.data_ptr() % 16 == 8
+        prob._workspace = shifted
+        c_s, g_s = prob.chi2_and_grad(theta, data, iv)
+        assert prob._workspace.data_ptr() == shifted.data_ptr()
+        assert torch.equal(c_s, c_j) and torch.equal(g_s, g_j)
+    prob.store_jacobian = False
+    c_r, g_r = prob.chi2_and_grad(theta, data, iv)
+    assert abs(float(c_j) - float(c_r)) <= 1e-11 * float(c_r)
+    sensitivity_check(g_j.cpu().numpy(), g_r.cpu().numpy(), theta, 1e-8, 1e-9,
+                      "jacobian vs recompute chi2 gradient")
+
+
 def test_adjoint_is_deterministic(setup):
     _, _, _, theta, v, prob = setup
     w = torch.as_tensor(np.random.default_rng(5).normal(size=v.shape[1]), device="cuda")

@@ -28,6 +28,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdint>
 #include <cstring>
 #include <tuple>
 #include <utility>
@@ -418,11 +419,18 @@ __global__ __launch_bounds__(256) void adjoint_reduce(const double* __restrict__
 // chi^2 and its gradient from a stored Jacobian: per ray g_r = 2 (I_r - d_r) w_r
 // with w_r = 1/sigma_r^2, grad_p = sum_r g_r J[p][r], chi^2 = sum_r (I_r - d_r)^2 w_r.
 // Block (x, y) takes rays [x R, x R + R) (R from jacobian_chunk_rays) and
-// parameters [y P, y P + P): thread t sums rays t, t + 256, ... of the range in
-// order, each wave sums its lanes with the xor butterfly, and the waves' sums are
-// added in wave order into partials[x][p] (entry n_params: chi^2, from the y = 0
-// blocks). adjoint_reduce then adds the partials of all x in a fixed order.
+// parameters [y P, y P + P). Thread t takes the ray pairs (r, r + 1) with
+// r = x R + 2 t + 512 i over the largest multiple of 512 rays of the range, then
+// single rays t, t + 256, ... of the rest, adding r before r + 1. Each wave sums its
+// lanes with the xor butterfly, and the waves' sums are added in wave order into
+// partials[x][p] (entry n_params: chi^2, from the y = 0 blocks). adjoint_reduce
+// then adds the partials of all x in a fixed order.
+//
+// VEC reads each pair's two Jacobian entries with one 16-byte load (J rows 16-byte
+// aligned: even leading dimension and base); otherwise two 8-byte loads. The
+// additions are the same either way, so the result depends on n_rays alone.
 constexpr int kJacParams = 32;
+constexpr int kJacPairStep = 512;  // rays per block iteration of the pair loop
 
 // Rays per block of jacobian_chi2: about 1024 blocks per parameter tile (enough
 // loads in flight to stream J at full rate), 256 to 8192 rays each. A function of
@@ -433,6 +441,7 @@ __host__ __device__ inline int64_t jacobian_chunk_rays(int64_t n_rays) {
   return r < 256 ? 256 : (r > 8192 ? 8192 : r);
 }
 
+template <bool VEC>
 __global__ __launch_bounds__(256) void jacobian_chi2(const double* __restrict__ jac, int64_t ld,
                                                      int n_params, int64_t n_rays,
                                                      int64_t chunk_rays,
@@ -448,16 +457,45 @@ __global__ __launch_bounds__(256) void jacobian_chi2(const double* __restrict__ 
   const int np = min(kJacParams, n_params - p0);
   const int64_t r0 = (int64_t)blockIdx.x * chunk_rays;
   const int64_t r1 = min(n_rays, r0 + chunk_rays);
+  const int64_t rv = r0 + (r1 - r0) / kJacPairStep * kJacPairStep;
+  const double* rows = jac + (int64_t)p0 * ld;
   double acc[kJacParams];
 #pragma unroll
   for (int q = 0; q < kJacParams; ++q) acc[q] = 0.0;
   double chi = 0.0;
-  for (int64_t r = r0 + t; r < r1; r += 256) {
+  for (int64_t r = r0 + 2 * t; r < rv; r += kJacPairStep) {
+    const double ra = vals[r] - target[r];
+    const double rb = vals[r + 1] - target[r + 1];
+    const double iva = inv_var != nullptr ? inv_var[r] : inv_var_scalar;
+    const double ivb = inv_var != nullptr ? inv_var[r + 1] : inv_var_scalar;
+    chi += ra * ra * iva;
+    chi += rb * rb * ivb;
+    const double ga = 2.0 * ra * iva;
+    const double gb = 2.0 * rb * ivb;
+    const double* col = rows + r;
+#pragma unroll
+    for (int q = 0; q < kJacParams; ++q) {
+      if (q < np) {
+        double ja, jb;
+        if constexpr (VEC) {
+          const double2 j = *reinterpret_cast<const double2*>(col + q * ld);
+          ja = j.x;
+          jb = j.y;
+        } else {
+          ja = col[q * ld];
+          jb = col[q * ld + 1];
+        }
+        acc[q] += ga * ja;
+        acc[q] += gb * jb;
+      }
+    }
+  }
+  for (int64_t r = rv + t; r < r1; r += 256) {
     const double res = vals[r] - target[r];
     const double iv = inv_var != nullptr ? inv_var[r] : inv_var_scalar;
     chi += res * res * iv;
     const double g = 2.0 * res * iv;
-    const double* col = jac + (int64_t)p0 * ld + r;
+    const double* col = rows + r;
 #pragma unroll
     for (int q = 0; q < kJacParams; ++q) {
       if (q < np) acc[q] += g * col[q * ld];
@@ -823,9 +861,12 @@ zodi_status zodi_adjoint(const zodi_model* model_host, const zodi_model* model_d
       st = check_hip(hipGetLastError(), "jacobian launch");
       if (st != ZODI_OK) return st;
       const dim3 g2((unsigned)chunks, (unsigned)((np + kJacParams - 1) / kJacParams));
-      hipLaunchKernelGGL(jacobian_chi2, g2, dim3(256), 0, (hipStream_t)stream,
-                         (const double*)workspace, n_rays, np, n_rays, chunk_rays,
-                         (const double*)vals, target, inv_var, inv_var_scalar, partials);
+      // 16-byte loads of J when its rows are 16-byte aligned (same additions either way)
+      const bool vec = (n_rays % 2) == 0 && ((uintptr_t)workspace % 16) == 0;
+      hipLaunchKernelGGL(vec ? jacobian_chi2<true> : jacobian_chi2<false>, g2, dim3(256), 0,
+                         (hipStream_t)stream, (const double*)workspace, n_rays, np, n_rays,
+                         chunk_rays, (const double*)vals, target, inv_var, inv_var_scalar,
+                         partials);
       st = check_hip(hipGetLastError(), "jacobian chi2 launch");
       if (st != ZODI_OK) return st;
     }
