@@ -199,6 +199,15 @@ __device__ __forceinline__ double pow20(double x) {
   return x16 * x4;
 }
 
+// exp(-x) for the bands' radial cut-off 1 - exp(-(R/delta_r)^20). For x >= 38,
+// exp(-x) < 2^-54 and 1 - exp(-x) rounds to 1 exactly, so 0 stands in for it; the
+// branch skips the exponential for the waves whose points all lie that far out.
+__device__ __forceinline__ double band_cutoff_exp(double x) {
+  double ex = 0.0;
+  if (x < 38.0) ex = exp_fast(-x);
+  return ex;
+}
+
 // Band, zodipy/number_density.py:76-110 (P[6] holds 3 n_0; the 1/v factor as in
 // the reference): 3 n_0/R exp(-zd^6) (1 + zd^p / v) (1 - exp(-(R/delta_r)^20)),
 // zd = zeta / delta_zeta.
@@ -211,7 +220,7 @@ __device__ __forceinline__ double band_density(ParamPtr P, uint32_t flags, const
   const double zd6 = zd4 * zd2;
   const double zdp = (flags & ZODI_CF_P_IS_4) ? zd4 : pow_pos(zd, p);
   const double term3 = 1.0 + zdp * inv_v;
-  const double term4 = 1.0 - exp_fast(-pow20(f.r * inv_dr));
+  const double term4 = 1.0 - band_cutoff_exp(pow20(f.r * inv_dr));
   return (three_n0 * f.inv_r) * exp_fast(-zd6) * term3 * term4;
 }
 

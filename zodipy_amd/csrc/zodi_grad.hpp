@@ -102,7 +102,7 @@ __device__ __forceinline__ DensGrad<5> band_grad(ParamPtr P, uint32_t flags, con
   const double t2 = exp_fast(-zd6);
   const double t3 = 1.0 + zdp * inv_v;
   const double x = pow20(f.r * inv_dr);
-  const double ex = exp_fast(-x);
+  const double ex = band_cutoff_exp(x);  // 0 where 1 - exp(-x) == 1
   const double t4 = 1.0 - ex;
   DensGrad<5> o;
   const double t24 = t2 * t4;

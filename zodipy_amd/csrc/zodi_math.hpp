@@ -47,9 +47,13 @@ __device__ __forceinline__ double exp_fast(double x) {
   constexpr double kLHi = 0x1.62e42fee00000p-7;         // ln2/64, 32 significant bits
   constexpr double kLLo = 0x1.a39ef35793c76p-39;        // ln2/64 - kLHi
   // Every exponent in the model is below ~50 (the positive ones are small
-  // temperature and power-law exponents); the lower clamp keeps the integer
-  // conversion in range and exp(-746) rounds to 0 (also for -inf). NaN passes.
-  const double xc = x < -746.0 ? -746.0 : x;
+  // temperature and power-law exponents). Arguments below -746 (-inf included)
+  // get the high word of -746, which leaves them in (-746.0005, -746]: the integer
+  // conversion stays in range and the result rounds to 0. One select of a 32-bit
+  // word instead of two. NaN passes.
+  const uint64_t xb = __builtin_bit_cast(uint64_t, x);
+  const uint32_t hi = x < -746.0 ? 0xC0875000u : (uint32_t)(xb >> 32);  // -746.0
+  const double xc = __builtin_bit_cast(double, ((uint64_t)hi << 32) | (uint32_t)xb);
   const double kd = __builtin_rint(xc * kInvL);
   double r = fma(-kd, kLHi, xc);
   r = fma(-kd, kLLo, r);
