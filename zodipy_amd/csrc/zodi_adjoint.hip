@@ -122,9 +122,10 @@ __device__ __forceinline__ void adjoint_group(Grp<Ks...>, std::integer_sequence<
     ((W[J - S] = h * gscale), ...);
   }
   double elon[N];
-  ((elon[J - S] = is_feature(kinds[J]) ? atan2(y.ey - M->comp[first + J][ZODI_P_Y0],
-                                               y.ex - M->comp[first + J][ZODI_P_X0])
-                                       : 0.0),
+  ((elon[J - S] = is_feature(kinds[J])
+                      ? zodi_trig::atan2_fast(y.ey - M->comp[first + J][ZODI_P_Y0],
+                                              y.ex - M->comp[first + J][ZODI_P_X0])
+                      : 0.0),
    ...);
   EarthTerm et[N];
   ((et[J - S] = is_feature(kinds[J]) ? earth_term(M->comp[first + J], y.ex, y.ey)

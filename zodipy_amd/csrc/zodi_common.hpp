@@ -293,10 +293,14 @@ __device__ __forceinline__ void group_pass(Grp<Ks...>, std::integer_sequence<int
   const double T0 = M->group_T0[g];
   const double mdelta = -M->group_delta[g];
 
+  // The Earth's longitude about the feature's centre, once per ray. atan2_fast
+  // keeps its coefficients in scalar registers; the runtime atan2's constants
+  // would stay in vector registers across the kernel (and spill).
   double elon[C];
-  ((elon[J] = is_feature(kinds[J]) ? atan2(y.ey - M->comp[first + J][ZODI_P_Y0],
-                                           y.ex - M->comp[first + J][ZODI_P_X0])
-                                   : 0.0),
+  ((elon[J] = is_feature(kinds[J])
+                  ? zodi_trig::atan2_fast(y.ey - M->comp[first + J][ZODI_P_Y0],
+                                          y.ex - M->comp[first + J][ZODI_P_X0])
+                  : 0.0),
    ...);
   bool scatter = false;
   if constexpr (INTEGRAND == ZODI_INTEGRAND_KELSALL) {

@@ -106,16 +106,17 @@ zodi_status validate_model(const zodi_model* m) {
 
 
 // Minimum waves per SIMD the register allocator must leave room for, per layout
-// (1 = no constraint). Measured on MI355X, the registry layouts run faster at 4
-// (128 VGPRs with a small spill) than at their unconstrained allocation: DIRBE
-// 167 VGPRs / 3 waves (5 % slower), RRM 190 VGPRs / 2 waves (15 % slower).
+// (1 = no constraint). Measured on MI355X: DIRBE and Kelsall4 run at 5 (95 and
+// 82 VGPRs, no spill; DIRBE 2.7 % faster than at 4, and 6 waves with a spill
+// gains nothing); RRM at 4 (128 VGPRs with a small spill), 15 % faster than its
+// unconstrained 190 VGPRs / 2 waves.
 template <class LAYOUT>
 struct ForwardOccupancy {
   static constexpr int value = 1;
 };
 template <>
 struct ForwardOccupancy<LayoutDirbe> {
-  static constexpr int value = 4;
+  static constexpr int value = 5;
 };
 template <>
 struct ForwardOccupancy<LayoutRRM> {
@@ -127,7 +128,7 @@ struct ForwardOccupancy<LayoutRRMSplit> {
 };
 template <>
 struct ForwardOccupancy<LayoutKelsall4> {
-  static constexpr int value = 4;
+  static constexpr int value = 5;
 };
 
 template <class LAYOUT, int BLOCK, int L>
@@ -184,7 +185,7 @@ __device__ __forceinline__ void generic_groups(const Ctx& cx, const FwdArgs& a, 
         const uint32_t flags = M->comp_flags[c];
         const int kind = M->comp_kind[c];
         const double elon =
-            is_feature(kind) ? atan2(y.ey - P[ZODI_P_Y0], y.ex - P[ZODI_P_X0]) : 0.0;
+            is_feature(kind) ? zodi_trig::atan2_fast(y.ey - P[ZODI_P_Y0], y.ex - P[ZODI_P_X0]) : 0.0;
         const double dens = density(kind, P, flags, nd.X0, nd.X1, nd.X2, nd.R, nd.inv_R, elon);
         sAcc[j][tid] += node_term<INTEGRAND>(P, flags, nd, wk, wB, dens, cx.solar);
       }
@@ -263,7 +264,8 @@ __global__ __launch_bounds__(256) void number_density_kernel(const zodi_model* m
   for (int c = 0; c < M->n_comps; ++c) {
     const ParamPtr P = M->comp[c];
     const int kind = M->comp_kind[c];
-    const double elon = is_feature(kind) ? atan2(ey - P[ZODI_P_Y0], ex - P[ZODI_P_X0]) : 0.0;
+    const double elon =
+        is_feature(kind) ? zodi_trig::atan2_fast(ey - P[ZODI_P_Y0], ex - P[ZODI_P_X0]) : 0.0;
     out[(int64_t)c * n + i] = density(kind, P, M->comp_flags[c], x0, x1, x2, R, inv_R, elon);
   }
 }
