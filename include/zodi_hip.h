@@ -122,7 +122,7 @@ enum { ZODI_S_C1 = 0, ZODI_S_C2 = 1, ZODI_S_C3 = 2, ZODI_S_SOLAR = 3,
  *   BAND         1/delta_zeta_rad, 1/v, 1/delta_r
  *   RING(_RRM)   1/sigma_r^2, 1/sigma_z, 1/sigma_r
  *   FEATURE(_RRM) 1/sigma_r^2, 1/sigma_z, 1/sigma_theta_rad^2, 1/sigma_r,
- *                1/sigma_theta_rad
+ *                1/sigma_theta_rad, cos(theta_rad), sin(theta_rad)
  *   FAN, COMET   1/Z_0
  *   NARROW_BAND  1/R_outer
  *   BROAD_BAND   1/sigma_bb, 1/R_outer

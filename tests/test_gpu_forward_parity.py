@@ -229,6 +229,35 @@ def test_number_density_kernel(gpu):
         assert_parity(got, want)
 
 
+def test_feature_longitude_axis_and_wrap(gpu):
+    """The feature's longitude offset on its axis (x' = y' = 0, with +0 and -0,
+    where the reference's atan2 of signed zeros gives 0 or +-pi) and at points
+    whose offset lies at the wrap (+-pi) or crosses it, for several Earth
+    longitudes including pi and -pi."""
+    from zodipy_amd.model_registry import model_registry
+    from zodipy_amd.number_density import density_device_model
+
+    zm = model_registry.get_model("dirbe")
+    om = OracleModel(25 * u.micron)
+    z = np.linspace(-0.3, 0.3, 7)
+    axis = np.concatenate([np.stack([np.zeros(7), np.zeros(7), z]),
+                           np.stack([-np.zeros(7), np.zeros(7), z]),
+                           np.stack([np.zeros(7), -np.zeros(7), z]),
+                           np.stack([-np.zeros(7), -np.zeros(7), z])], axis=1)
+    for earth in ([1.0, 0.0, 0.0], [-1.0, 0.0, 0.0], [-1.0, -0.0, 0.0], [0.3, -0.95, 0.0]):
+        earth = np.array(earth)
+        lon_e = np.arctan2(earth[1], earth[0])
+        theta_f = np.radians(-10.0)  # the DIRBE feature's offset
+        # points 1 AU out at the longitude where the offset is pi, and 1e-9 rad to
+        # either side of it
+        lon = lon_e + theta_f + np.pi + np.array([-1e-9, 0.0, 1e-9, 3.0, -3.0])
+        ring = np.stack([np.cos(lon), np.sin(lon), np.full(lon.size, 0.01)])
+        X = np.concatenate([axis, ring], axis=1)
+        got = density_device_model(zm).number_density(X, earth).cpu().numpy()
+        want = om.number_density(X, earth)
+        assert_parity(got, want, rtol=1e-12)
+
+
 def test_device_tensor_output_and_stream(gpu):
     v = torch.as_tensor(random_dirs(4096, seed=30), device="cuda")
     obs = np.array([1.0, 0.0, 0.0])

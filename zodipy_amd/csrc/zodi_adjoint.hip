@@ -121,11 +121,9 @@ __device__ __forceinline__ void adjoint_group(Grp<Ks...>, std::integer_sequence<
   } else {
     ((W[J - S] = h * gscale), ...);
   }
-  double elon[N];
-  ((elon[J - S] = is_feature(kinds[J])
-                      ? zodi_trig::atan2_fast(y.ey - M->comp[first + J][ZODI_P_Y0],
-                                              y.ex - M->comp[first + J][ZODI_P_X0])
-                      : 0.0),
+  FeatureDir fdir[N];
+  ((fdir[J - S] = is_feature(kinds[J]) ? feature_dir(M->comp[first + J], y.ex, y.ey)
+                                       : FeatureDir{1.0, 0.0}),
    ...);
   EarthTerm et[N];
   ((et[J - S] = is_feature(kinds[J]) ? earth_term(M->comp[first + J], y.ex, y.ey)
@@ -177,7 +175,7 @@ __device__ __forceinline__ void adjoint_group(Grp<Ks...>, std::integer_sequence<
           const ParamPtr P = Mk->comp[first + J];
           const uint32_t flags = Mk->comp_flags[first + J];
           const CompFrame f = comp_frame(P, flags, nd.X0, nd.X1, nd.X2, nd.R, nd.inv_R);
-          const DensGrad<NP> dg = density_grad<KIND>(P, flags, f, elon[J - S], logR);
+          const DensGrad<NP> dg = density_grad<KIND>(P, flags, f, fdir[J - S], logR);
           auto& A = std::get<J - S>(acc).v;
           const double wn = wk * W[J - S];
           if constexpr (INTEGRAND == ZODI_INTEGRAND_KELSALL) {

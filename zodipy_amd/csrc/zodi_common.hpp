@@ -254,7 +254,7 @@ inline LayoutId match_layout(const zodi_model* m) {
 // Density of a component whose kind is known at compile time.
 template <int KIND>
 __device__ __forceinline__ double density_k(ParamPtr P, uint32_t flags, const Node& nd,
-                                            double earth_lon) {
+                                            const FeatureDir& fd) {
   if constexpr (KIND == ZODI_INTERSTELLAR) {
     return P[6];
   } else {
@@ -262,13 +262,13 @@ __device__ __forceinline__ double density_k(ParamPtr P, uint32_t flags, const No
     if constexpr (KIND == ZODI_CLOUD) return cloud_density(P, f);
     if constexpr (KIND == ZODI_BAND) return band_density(P, flags, f);
     if constexpr (KIND == ZODI_RING) return ring_density(P, f);
-    if constexpr (KIND == ZODI_FEATURE) return feature_density(P, f, earth_lon);
+    if constexpr (KIND == ZODI_FEATURE) return feature_density(P, f, fd);
     if constexpr (KIND == ZODI_FAN) return fan_density(P, f);
     if constexpr (KIND == ZODI_COMET) return comet_density(P, f);
     if constexpr (KIND == ZODI_NARROW_BAND) return narrow_band_density(P, f);
     if constexpr (KIND == ZODI_BROAD_BAND) return broad_band_density(P, f);
     if constexpr (KIND == ZODI_RING_RRM) return P[10] * ring_density(P, f);
-    if constexpr (KIND == ZODI_FEATURE_RRM) return P[12] * feature_density(P, f, earth_lon);
+    if constexpr (KIND == ZODI_FEATURE_RRM) return P[12] * feature_density(P, f, fd);
   }
 }
 
@@ -293,14 +293,10 @@ __device__ __forceinline__ void group_pass(Grp<Ks...>, std::integer_sequence<int
   const double T0 = M->group_T0[g];
   const double mdelta = -M->group_delta[g];
 
-  // The Earth's longitude about the feature's centre, once per ray. atan2_fast
-  // keeps its coefficients in scalar registers; the runtime atan2's constants
-  // would stay in vector registers across the kernel (and spill).
-  double elon[C];
-  ((elon[J] = is_feature(kinds[J])
-                  ? zodi_trig::atan2_fast(y.ey - M->comp[first + J][ZODI_P_Y0],
-                                          y.ex - M->comp[first + J][ZODI_P_X0])
-                  : 0.0),
+  // The feature's centre line, once per ray.
+  FeatureDir fdir[C];
+  ((fdir[J] = is_feature(kinds[J]) ? feature_dir(M->comp[first + J], y.ex, y.ey)
+                                   : FeatureDir{1.0, 0.0}),
    ...);
   bool scatter = false;
   if constexpr (INTEGRAND == ZODI_INTEGRAND_KELSALL) {
@@ -322,7 +318,7 @@ __device__ __forceinline__ void group_pass(Grp<Ks...>, std::integer_sequence<int
     const double wB = wk * nd.B;
     ((acc[J] += node_term<INTEGRAND>(
           Mk->comp[first + J], Mk->comp_flags[first + J], nd, wk, wB,
-          density_k<kinds[J]>(Mk->comp[first + J], Mk->comp_flags[first + J], nd, elon[J]),
+          density_k<kinds[J]>(Mk->comp[first + J], Mk->comp_flags[first + J], nd, fdir[J]),
           Mk->shared[ZODI_S_SOLAR])),
      ...);
   }

@@ -231,15 +231,55 @@ __device__ __forceinline__ double ring_density(ParamPtr P, const CompFrame& f) {
   return n0 * exp_fast(-(d * d) * P[13] - fabs(f.zc) * P[14]);
 }
 
+// The feature's centre line for one ray: (cos psi, sin psi) with
+// psi = theta_E + theta_f, theta_E = atan2(E'_y, E'_x) the Earth's longitude about
+// the component centre (E' = X_earth - X_0) and theta_f the feature's offset
+// (cos and sin in derived slots 18 and 19).
+struct FeatureDir {
+  double c, s;
+};
+
+__device__ __forceinline__ FeatureDir feature_dir(ParamPtr P, double ex, double ey) {
+  const double epx = ex - P[ZODI_P_X0], epy = ey - P[ZODI_P_Y0];
+  const double rho2 = epx * epx + epy * epy;
+  double ce, se;
+  if (rho2 == 0.0) {  // atan2(+-0, +-0) = 0 or +-pi
+    ce = __builtin_signbit(epx) ? -1.0 : 1.0;
+    se = 0.0;
+  } else {
+    double rho, inv_rho;
+    norm_and_inverse(rho2, rho, inv_rho);
+    ce = epx * inv_rho;
+    se = epy * inv_rho;
+  }
+  const double ct = P[ZODI_P_D0 + 5], st = P[ZODI_P_D0 + 6];
+  return {ce * ct - se * st, se * ct + ce * st};
+}
+
+// dtheta = theta_comp - theta_f wrapped into [-pi, pi) with a floor modulo, as
+// zodipy/number_density.py:170-173 forms it, theta_comp = atan2(y', x') - theta_E.
+// It is the longitude of (x', y') about the centre line: the atan2 of the point
+// rotated by -psi, which lies in (-pi, pi] and agrees up to rounding (and the
+// sign at +-pi, where dtheta^2 is the same). On the axis (x' = y' = 0) the
+// reference's atan2(+-0, +-0) is 0 (x' = +0) or +-pi (x' = -0), so dtheta is the
+// angle of (cos psi, -sin psi) or of (-cos psi, sin psi).
+__device__ __forceinline__ double feature_dtheta(const CompFrame& f, const FeatureDir& d) {
+  double xr = f.x * d.c + f.y * d.s;
+  double yr = f.y * d.c - f.x * d.s;
+  if (f.x == 0.0 && f.y == 0.0) {
+    const double sg = __builtin_signbit(f.x) ? -1.0 : 1.0;
+    xr = sg * d.c;
+    yr = -sg * d.s;
+  }
+  return zodi_trig::atan2_fast(yr, xr);
+}
+
 // Feature, zodipy/number_density.py:142-181: n_0 exp(-((R - R_f)^2/sigma_r^2 +
-// |Z|/sigma_z + dtheta^2/sigma_theta^2)), dtheta wrapped with a floor modulo;
-// `earth_lon` = atan2(E'_y, E'_x) with E' = X_earth - X_0.
+// |Z|/sigma_z + dtheta^2/sigma_theta^2)).
 __device__ __forceinline__ double feature_density(ParamPtr P, const CompFrame& f,
-                                                  double earth_lon) {
-  const double n0 = P[6], Rf = P[7], th = P[10];
-  const double theta_comp = zodi_trig::atan2_fast(f.y, f.x) - earth_lon;
-  double dth = theta_comp - th;
-  dth = floor_mod_pos(dth + kPi, kTwoPi) - kPi;
+                                                  const FeatureDir& fd) {
+  const double n0 = P[6], Rf = P[7];
+  const double dth = feature_dtheta(f, fd);
   const double d = f.r - Rf;
   double e = (d * d) * P[13];
   e += fabs(f.zc) * P[14];
@@ -296,7 +336,7 @@ __host__ __device__ constexpr bool is_feature(int kind) {
 // Density of component `kind` at heliocentric X (|X| = R, 1/|X| = inv_R).
 __device__ __forceinline__ double density(int kind, ParamPtr P, uint32_t flags, double X0,
                                           double X1, double X2, double R, double inv_R,
-                                          double earth_lon) {
+                                          const FeatureDir& fd) {
   if (kind == ZODI_INTERSTELLAR) return P[6];  // zodipy/number_density.py:259-264
   const CompFrame f = comp_frame(P, flags, X0, X1, X2, R, inv_R);
   switch (kind) {
@@ -307,7 +347,7 @@ __device__ __forceinline__ double density(int kind, ParamPtr P, uint32_t flags, 
     case ZODI_RING:
       return ring_density(P, f);
     case ZODI_FEATURE:
-      return feature_density(P, f, earth_lon);
+      return feature_density(P, f, fd);
     case ZODI_FAN:
       return fan_density(P, f);
     case ZODI_COMET:
@@ -319,7 +359,7 @@ __device__ __forceinline__ double density(int kind, ParamPtr P, uint32_t flags, 
     case ZODI_RING_RRM:  // zodipy/number_density.py:345-370
       return P[10] * ring_density(P, f);
     case ZODI_FEATURE_RRM:  // zodipy/number_density.py:373-404
-      return P[12] * feature_density(P, f, earth_lon);
+      return P[12] * feature_density(P, f, fd);
     default:
       return __builtin_nan("");
   }

@@ -184,9 +184,8 @@ __device__ __forceinline__ void generic_groups(const Ctx& cx, const FwdArgs& a, 
         const ParamPtr P = M->comp[c];
         const uint32_t flags = M->comp_flags[c];
         const int kind = M->comp_kind[c];
-        const double elon =
-            is_feature(kind) ? zodi_trig::atan2_fast(y.ey - P[ZODI_P_Y0], y.ex - P[ZODI_P_X0]) : 0.0;
-        const double dens = density(kind, P, flags, nd.X0, nd.X1, nd.X2, nd.R, nd.inv_R, elon);
+        const FeatureDir fd = is_feature(kind) ? feature_dir(P, y.ex, y.ey) : FeatureDir{1.0, 0.0};
+        const double dens = density(kind, P, flags, nd.X0, nd.X1, nd.X2, nd.R, nd.inv_R, fd);
         sAcc[j][tid] += node_term<INTEGRAND>(P, flags, nd, wk, wB, dens, cx.solar);
       }
     }
@@ -258,15 +257,19 @@ __global__ __launch_bounds__(256) void number_density_kernel(const zodi_model* m
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const double x0 = X[i], x1 = X[n + i], x2 = X[2 * n + i];
+  const double R2 = x0 * x0 + x1 * x1 + x2 * x2;
   double R, inv_R;
-  norm_and_inverse(x0 * x0 + x1 * x1 + x2 * x2, R, inv_R);
+  norm_and_inverse(R2, R, inv_R);
+  if (R2 == 0.0) {  // the Sun's centre (a grid may hold it): R = 0 as sqrt gives it
+    R = 0.0;
+    inv_R = __builtin_inf();
+  }
   const double ex = earth[0], ey = earth[1];
   for (int c = 0; c < M->n_comps; ++c) {
     const ParamPtr P = M->comp[c];
     const int kind = M->comp_kind[c];
-    const double elon =
-        is_feature(kind) ? zodi_trig::atan2_fast(ey - P[ZODI_P_Y0], ex - P[ZODI_P_X0]) : 0.0;
-    out[(int64_t)c * n + i] = density(kind, P, M->comp_flags[c], x0, x1, x2, R, inv_R, elon);
+    const FeatureDir fd = is_feature(kind) ? feature_dir(P, ex, ey) : FeatureDir{1.0, 0.0};
+    out[(int64_t)c * n + i] = density(kind, P, M->comp_flags[c], x0, x1, x2, R, inv_R, fd);
   }
 }
 

@@ -150,12 +150,10 @@ __device__ __forceinline__ DensGrad<4> ring_grad(ParamPtr P, const CompFrame& f)
 
 // Feature: n_0, R, sigma_r, sigma_z, theta [deg], sigma_theta [deg].
 __device__ __forceinline__ DensGrad<6> feature_grad(ParamPtr P, const CompFrame& f,
-                                                    double earth_lon) {
-  const double n0 = P[6], Rf = P[7], th = P[10];
+                                                    const FeatureDir& fd) {
+  const double n0 = P[6], Rf = P[7];
   const double inv_sr2 = P[13], inv_sz = P[14], inv_sth2 = P[15], inv_sr = P[16], inv_sth = P[17];
-  const double theta_comp = zodi_trig::atan2_fast(f.y, f.x) - earth_lon;
-  double dth = theta_comp - th;
-  dth = floor_mod_pos(dth + kPi, kTwoPi) - kPi;
+  const double dth = feature_dtheta(f, fd);
   const double d = f.r - Rf;
   const double za = fabs(f.zc);
   double e = (d * d) * inv_sr2;
@@ -337,7 +335,7 @@ __device__ __forceinline__ DensGrad<6> broad_band_grad(ParamPtr P, const CompFra
 template <int KIND>
 __device__ __forceinline__ DensGrad<kind_nparams<KIND>()> density_grad(ParamPtr P, uint32_t flags,
                                                                         const CompFrame& f,
-                                                                        double earth_lon,
+                                                                        const FeatureDir& fd,
                                                                         double logR) {
   if constexpr (KIND == ZODI_CLOUD) {
     return cloud_grad(P, flags, f, logR);
@@ -346,7 +344,7 @@ __device__ __forceinline__ DensGrad<kind_nparams<KIND>()> density_grad(ParamPtr 
   } else if constexpr (KIND == ZODI_RING) {
     return ring_grad(P, f);
   } else if constexpr (KIND == ZODI_FEATURE) {
-    return feature_grad(P, f, earth_lon);
+    return feature_grad(P, f, fd);
   } else if constexpr (KIND == ZODI_FAN) {
     return fan_grad(P, f);
   } else if constexpr (KIND == ZODI_COMET) {
@@ -374,7 +372,7 @@ __device__ __forceinline__ DensGrad<kind_nparams<KIND>()> density_grad(ParamPtr 
     return o;
   } else {
     static_assert(KIND == ZODI_FEATURE_RRM, "no derivatives for this component kind");
-    const DensGrad<6> r = feature_grad(P, f, earth_lon);
+    const DensGrad<6> r = feature_grad(P, f, fd);
     const double A = P[12];
     DensGrad<7> o;
     o.n = A * r.n;

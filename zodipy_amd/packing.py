@@ -129,7 +129,8 @@ def _derived_slots(comp: ZodiacalComponent, kind: int) -> list[float]:
         return [1.0 / c.sigma_r**2, 1.0 / c.sigma_z, 1.0 / c.sigma_r]
     if kind in (KIND_FEATURE, KIND_FEATURE_RRM):
         return [1.0 / c.sigma_r**2, 1.0 / c.sigma_z, 1.0 / c.sigma_theta_rad**2,
-                1.0 / c.sigma_r, 1.0 / c.sigma_theta_rad]
+                1.0 / c.sigma_r, 1.0 / c.sigma_theta_rad, float(np.cos(c.theta_rad)),
+                float(np.sin(c.theta_rad))]
     if kind in (KIND_FAN, KIND_COMET):
         return [1.0 / c.Z_0]
     if kind == KIND_NARROW_BAND:
