@@ -131,6 +131,7 @@ struct Tables {
   int n_t;
 };
 
+template <bool NANFIX = true>
 __device__ __forceinline__ Node node_geometry(const Ray& y, double h, double mid, double xk,
                                               double T0, double mdelta, const Tables& tb) {
   Node nd;
@@ -141,7 +142,7 @@ __device__ __forceinline__ Node node_geometry(const Ray& y, double h, double mid
   norm_and_inverse(nd.X0 * nd.X0 + nd.X1 * nd.X1 + nd.X2 * nd.X2, nd.R, nd.inv_R);
   // T = T0 R^-delta (zodipy/blackbody.py:16-30), B = interp(T) (zodipy/brightness.py:48)
   const double T = T0 * pow_radius(nd.R, mdelta);
-  nd.B = planck_lookup(T, tb.tab, tb.n_t, tb.T_first, tb.T_last, tb.inv_step);
+  nd.B = planck_lookup<NANFIX>(T, tb.tab, tb.n_t, tb.T_first, tb.T_last, tb.inv_step);
   nd.phi = 0.0;
   return nd;
 }
@@ -308,7 +309,10 @@ __device__ __forceinline__ void group_pass(Grp<Ks...>, std::integer_sequence<int
     const ModelPtr Mk = opaque(M);
     const double xk = Mk->gl_x[k];
     const double wk = Mk->gl_w[k];
-    Node nd = node_geometry(y, h, mid, xk, T0, mdelta, cx.tb);
+    // Only the interstellar density does not depend on the position (and so on a
+    // NaN temperature's cause); without it the lookup skips its NaN pass-through.
+    constexpr bool kConstDensity = ((kinds[J] == ZODI_INTERSTELLAR) || ...);
+    Node nd = node_geometry<kConstDensity>(y, h, mid, xk, T0, mdelta, cx.tb);
     if constexpr (INTEGRAND == ZODI_INTEGRAND_KELSALL) {
       if (scatter) {
         nd.phi = phase_function(h * xk + mid, y.ux, y.uy, y.uz, nd.X0, nd.X1, nd.X2, nd.R,

@@ -142,6 +142,10 @@ __device__ __forceinline__ double sphere_distance(const RayGeom& g, double cutof
 // it one interval off (T within ~1e-14 relative of a grid temperature), the
 // neighbouring segment is evaluated at a point it meets within rounding, so the
 // value agrees to ~1e-15. Each LDS entry is (T_j, slope_j, B_j, 0).
+// NANFIX passes a NaN temperature through (np.interp returns NaN for it). A NaN
+// T comes from a NaN position, which also makes every density that depends on
+// the position NaN: callers whose densities all do may leave the check out.
+template <bool NANFIX = true>
 __device__ __forceinline__ double planck_lookup(double T, const double* __restrict__ tab, int n,
                                                 double T_first, double T_last, double inv_step) {
   const double t = fmin(fmax(T, T_first), T_last);
@@ -149,7 +153,8 @@ __device__ __forceinline__ double planck_lookup(double T, const double* __restri
   j = j > n - 2 ? n - 2 : j;
   const double* e = tab + 4 * j;
   const double B = e[1] * (t - e[0]) + e[2];
-  return T == T ? B : T;
+  if constexpr (NANFIX) return T == T ? B : T;
+  return B;
 }
 
 // Position in a component's frame: offset by X_0 unless the component is

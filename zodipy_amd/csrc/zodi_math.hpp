@@ -1,9 +1,10 @@
 // Table-driven FP64 exp and log for the zodiacal kernels (gfx950).
 //
-// exp: Tang's table-driven method (ACM TOMS 15(2):144-157, 1989). x = (64 m + j) ln2/64 + r,
-//   |r| <= ln2/128; exp(x) = 2^m 2^(j/64) (1 + expm1(r)) with expm1(r) from its Taylor
-//   series to r^6 (truncation < 3e-20). About 1.5 ulp; exactly rounded 1 + expm1(r)
-//   for |x| < ln2/128, which keeps 1 - exp(-x) faithful for small x.
+// exp: Tang's table-driven method (ACM TOMS 15(2):144-157, 1989). x = (128 m + j) ln2/128 + r,
+//   |r| <= ln2/256; exp(x) = 2^m 2^(j/128) (1 + expm1(r)) with expm1(r) from its Taylor
+//   series to r^5 (truncation < 6e-19). Within 1 ulp of the exact value (checked
+//   against mpmath on 5.5e5 arguments); exactly rounded 1 + expm1(r) for |x| < ln2/256,
+//   which keeps 1 - exp(-x) faithful for small x.
 // log: Tang's table-driven method (ACM TOMS 16(4):378-400, 1990). x = 2^e m, m in [1/2, 1),
 //   c_j = 1/2 + j/256 with j the top 7 mantissa bits, r = (m - c_j)/c_j in [0, 1/128);
 //   log(x) = e ln2 + log(c_j) + log1p(r) with log1p from its Taylor series to r^7
@@ -22,30 +23,30 @@
 
 namespace zodi {
 
-__constant__ double kMathTablesConst[64 + 128 + 128] = {ZODI_EXP2_TABLE, ZODI_LOG_INV_TABLE,
-                                                        ZODI_LOG_C_TABLE};
-// Taylor coefficients: expm1 r^6..r^3 (1/720, 1/120, 1/24, 1/6); log1p r^7..r^3.
-__constant__ double kExpPoly[4] = {1.0 / 720.0, 1.0 / 120.0, 1.0 / 24.0, 1.0 / 6.0};
+__constant__ double kMathTablesConst[128 + 128 + 128] = {ZODI_EXP2_TABLE, ZODI_LOG_INV_TABLE,
+                                                         ZODI_LOG_C_TABLE};
+// Taylor coefficients: expm1 r^5..r^3 (1/120, 1/24, 1/6); log1p r^7..r^3.
+__constant__ double kExpPoly[3] = {1.0 / 120.0, 1.0 / 24.0, 1.0 / 6.0};
 __constant__ double kLogPoly[5] = {1.0 / 7.0, -1.0 / 6.0, 1.0 / 5.0, -1.0 / 4.0, 1.0 / 3.0};
 
-__shared__ double g_exp2[64];
+__shared__ double g_exp2[128];
 __shared__ double g_log_inv[128];
 __shared__ double g_log_c[128];
 
 // Copy the tables to LDS; every thread of the block must call this before use.
 __device__ __forceinline__ void stage_math_tables() {
-  for (int i = threadIdx.x; i < 64; i += blockDim.x) g_exp2[i] = kMathTablesConst[i];
   for (int i = threadIdx.x; i < 128; i += blockDim.x) {
-    g_log_inv[i] = kMathTablesConst[64 + i];
-    g_log_c[i] = kMathTablesConst[192 + i];
+    g_exp2[i] = kMathTablesConst[i];
+    g_log_inv[i] = kMathTablesConst[128 + i];
+    g_log_c[i] = kMathTablesConst[256 + i];
   }
   __syncthreads();
 }
 
 __device__ __forceinline__ double exp_fast(double x) {
-  constexpr double kInvL = 92.33248261689366;           // 64 / ln 2
-  constexpr double kLHi = 0x1.62e42fee00000p-7;         // ln2/64, 32 significant bits
-  constexpr double kLLo = 0x1.a39ef35793c76p-39;        // ln2/64 - kLHi
+  constexpr double kInvL = 0x1.71547652b82fep+7;        // 128 / ln 2
+  constexpr double kLHi = 0x1.62e42fee00000p-8;         // ln2/128, 32 significant bits
+  constexpr double kLLo = 0x1.a39ef35793c76p-40;        // ln2/128 - kLHi
   // Every exponent in the model is below ~50 (the positive ones are small
   // temperature and power-law exponents). Arguments below -746 (-inf included)
   // get the high word of -746, which leaves them in (-746.0005, -746]: the integer
@@ -58,11 +59,10 @@ __device__ __forceinline__ double exp_fast(double x) {
   double r = fma(-kd, kLHi, xc);
   r = fma(-kd, kLLo, r);
   const int k = (int)kd;
-  const int j = k & 63;
-  const int m = k >> 6;
+  const int j = k & 127;
+  const int m = k >> 7;
   double p = fma(kExpPoly[0], r, kExpPoly[1]);
   p = fma(p, r, kExpPoly[2]);
-  p = fma(p, r, kExpPoly[3]);
   p = fma(p, r, 0.5);
   const double q = fma(p * r, r, r);  // expm1(r)
   const double t = g_exp2[j];
