@@ -6,9 +6,9 @@
 //   against mpmath on 5.5e5 arguments); exactly rounded 1 + expm1(r) for |x| < ln2/256,
 //   which keeps 1 - exp(-x) faithful for small x.
 // log: Tang's table-driven method (ACM TOMS 16(4):378-400, 1990). x = 2^e m, m in [1/2, 1),
-//   c_j = 1/2 + j/256 with j the top 7 mantissa bits, r = (m - c_j)/c_j in [0, 1/128);
-//   log(x) = e ln2 + log(c_j) + log1p(r) with log1p from its Taylor series to r^7
-//   (truncation < 2e-18). Absolute error ~1e-16 + 0.5 ulp: the kernels only use log
+//   c_j = 1/2 + j/512 with j the top 8 mantissa bits, r = (m - c_j)/c_j in [0, 1/256);
+//   log(x) = e ln2 + log(c_j) + log1p(r) with log1p from its Taylor series to r^6
+//   (truncation < 3e-18). Absolute error ~1e-16 + 0.5 ulp: the kernels only use log
 //   inside exp(y log x), where the absolute error is what matters.
 //
 // Tables (correctly rounded, gen_math_tables.py) are staged in LDS by every kernel
@@ -23,22 +23,22 @@
 
 namespace zodi {
 
-__constant__ double kMathTablesConst[128 + 128 + 128] = {ZODI_EXP2_TABLE, ZODI_LOG_INV_TABLE,
+__constant__ double kMathTablesConst[128 + 256 + 256] = {ZODI_EXP2_TABLE, ZODI_LOG_INV_TABLE,
                                                          ZODI_LOG_C_TABLE};
-// Taylor coefficients: expm1 r^5..r^3 (1/120, 1/24, 1/6); log1p r^7..r^3.
+// Taylor coefficients: expm1 r^5..r^3 (1/120, 1/24, 1/6); log1p r^6..r^3.
 __constant__ double kExpPoly[3] = {1.0 / 120.0, 1.0 / 24.0, 1.0 / 6.0};
-__constant__ double kLogPoly[5] = {1.0 / 7.0, -1.0 / 6.0, 1.0 / 5.0, -1.0 / 4.0, 1.0 / 3.0};
+__constant__ double kLogPoly[4] = {-1.0 / 6.0, 1.0 / 5.0, -1.0 / 4.0, 1.0 / 3.0};
 
 __shared__ double g_exp2[128];
-__shared__ double g_log_inv[128];
-__shared__ double g_log_c[128];
+__shared__ double g_log_inv[256];
+__shared__ double g_log_c[256];
 
 // Copy the tables to LDS; every thread of the block must call this before use.
 __device__ __forceinline__ void stage_math_tables() {
-  for (int i = threadIdx.x; i < 128; i += blockDim.x) {
-    g_exp2[i] = kMathTablesConst[i];
+  for (int i = threadIdx.x; i < 128; i += blockDim.x) g_exp2[i] = kMathTablesConst[i];
+  for (int i = threadIdx.x; i < 256; i += blockDim.x) {
     g_log_inv[i] = kMathTablesConst[128 + i];
-    g_log_c[i] = kMathTablesConst[256 + i];
+    g_log_c[i] = kMathTablesConst[384 + i];
   }
   __syncthreads();
 }
@@ -76,13 +76,12 @@ __device__ __forceinline__ double log_pos(double x) {
   const int e = __builtin_amdgcn_frexp_exp(x);
   const double m = __builtin_amdgcn_frexp_mant(x);  // [1/2, 1) for finite x != 0
   const uint32_t hi = (uint32_t)(__builtin_bit_cast(uint64_t, m) >> 32);
-  const int j = (int)((hi >> 13) & 127u);
-  const double c = fma((double)j, 1.0 / 256.0, 0.5);
+  const int j = (int)((hi >> 12) & 255u);
+  const double c = fma((double)j, 1.0 / 512.0, 0.5);
   const double r = (m - c) * g_log_inv[j];  // m - c is exact
   double p = fma(kLogPoly[0], r, kLogPoly[1]);
   p = fma(p, r, kLogPoly[2]);
   p = fma(p, r, kLogPoly[3]);
-  p = fma(p, r, kLogPoly[4]);
   p = fma(p, r, -0.5);
   const double l1p = fma(p * r, r, r);  // log1p(r)
   const double de = (double)e;
