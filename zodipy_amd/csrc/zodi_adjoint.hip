@@ -153,7 +153,7 @@ __device__ __forceinline__ void adjoint_group(Grp<Ks...>, std::integer_sequence<
     nd.X0 = s_los * y.ux + y.ox;
     nd.X1 = s_los * y.uy + y.oy;
     nd.X2 = s_los * y.uz + y.oz;
-    norm_and_inverse(nd.X0 * nd.X0 + nd.X1 * nd.X1 + nd.X2 * nd.X2, nd.R, nd.inv_R);
+    norm_and_inverse_node(nd.X0 * nd.X0 + nd.X1 * nd.X1 + nd.X2 * nd.X2, nd.R, nd.inv_R);
     const double logR = log_pos(nd.R);
     const double T = T0 * exp_fast(mdelta * logR);
     double dBdT;

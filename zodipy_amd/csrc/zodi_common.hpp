@@ -139,7 +139,7 @@ __device__ __forceinline__ Node node_geometry(const Ray& y, double h, double mid
   nd.X0 = s_los * y.ux + y.ox;
   nd.X1 = s_los * y.uy + y.oy;
   nd.X2 = s_los * y.uz + y.oz;
-  norm_and_inverse(nd.X0 * nd.X0 + nd.X1 * nd.X1 + nd.X2 * nd.X2, nd.R, nd.inv_R);
+  norm_and_inverse_node(nd.X0 * nd.X0 + nd.X1 * nd.X1 + nd.X2 * nd.X2, nd.R, nd.inv_R);
   // T = T0 R^-delta (zodipy/blackbody.py:16-30), B = interp(T) (zodipy/brightness.py:48)
   const double T = T0 * pow_radius(nd.R, mdelta);
   nd.B = planck_lookup<NANFIX>(T, tb.tab, tb.n_t, tb.T_first, tb.T_last, tb.inv_step);

@@ -63,6 +63,7 @@ __device__ __forceinline__ double pow_radius(double x, double y) {
 
 // |X| and 1/|X| from |X|^2: a hardware reciprocal square root refined by two
 // Newton steps (y <- y (3 - a y^2) / 2), then |X| = a y. Both within ~1 ulp.
+// For per-ray and per-point quantities.
 __device__ __forceinline__ void norm_and_inverse(double a, double& r, double& inv_r) {
   double y = __builtin_amdgcn_rsq(a);
   double t = a * y;
@@ -77,6 +78,20 @@ __device__ __forceinline__ void norm_and_inverse(double a, double& r, double& in
   y = fma(y, e, y);
   r = t;
   inv_r = y;
+}
+
+// The same for the quadrature nodes, with one Newton step: the hardware
+// estimate's relative error is below 2^-24 (measured on gfx950 over 2^20
+// arguments spanning 2^-20..2^20), and one coupled step y <- y (1 + e),
+// t <- t (1 + e), e = (1 - a y^2)/2, t = a y, leaves 1.5 eps^2 < 5e-15 for both.
+// Every node quantity then moves by about 1e-14 relative or less, which the
+// integrals' 1e-10 parity bound leaves untouched.
+__device__ __forceinline__ void norm_and_inverse_node(double a, double& r, double& inv_r) {
+  const double y = __builtin_amdgcn_rsq(a);
+  const double t = a * y;
+  const double e = fma(-t, 0.5 * y, 0.5);
+  inv_r = fma(y, e, y);
+  r = fma(t, e, t);
 }
 
 // Python/NumPy float remainder a % b for b > 0 (numpy npy_divmod): the exact
@@ -178,7 +193,7 @@ __device__ __forceinline__ CompFrame comp_frame(ParamPtr P, uint32_t flags, doub
     f.x = X0 - P[ZODI_P_X0];
     f.y = X1 - P[ZODI_P_Y0];
     f.z = X2 - P[ZODI_P_Z0];
-    norm_and_inverse(f.x * f.x + f.y * f.y + f.z * f.z, f.r, f.inv_r);
+    norm_and_inverse_node(f.x * f.x + f.y * f.y + f.z * f.z, f.r, f.inv_r);
   }
   f.zc = f.x * P[ZODI_P_NX] - f.y * P[ZODI_P_NY] + f.z * P[ZODI_P_NZ];
   return f;
