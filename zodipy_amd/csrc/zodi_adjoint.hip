@@ -140,8 +140,12 @@ __device__ __forceinline__ void adjoint_group(Grp<Ks...>, std::integer_sequence<
   const double phi_const = M->shared[ZODI_S_PHASE_NORM] * (M->shared[ZODI_S_C1] + 1.0);
 
   std::tuple<CompAcc<kinds[J]>...> acc{};
-  double Iv[N];  // JAC: node sums of each component's integrand, as the forward forms them
-  ((Iv[J - S] = 0.0), ...);
+  // JAC: node sums per component, Ib = sum w n B and (Kelsall) Is = sum w n F Phi / R^2.
+  // The component's integral, and its emissivity and albedo partials (and RRM's
+  // calibration partial), are linear in them: they are formed after the loop.
+  double Ib[N], Is[N];
+  ((Ib[J - S] = 0.0), ...);
+  ((Is[J - S] = 0.0), ...);
 
   const int K = y.active ? cx.K : 0;
   for (int k = sub; k < K; k += SPLIT) {
@@ -185,16 +189,18 @@ __device__ __forceinline__ void adjoint_group(Grp<Ks...>, std::integer_sequence<
             const double scat_unit = (solar * inv_R2) * pg.phi;  // F Phi / R^2
             double e = oma * (eps * nd.B);
             if (flags & ZODI_CF_SCATTER) e += alb * scat_unit;
-            if constexpr (JAC) {
-              Iv[J - S] += (flags & ZODI_CF_SCATTER) ? wk * (e * dg.n) : (wk * nd.B) * dg.n;
-            }
             const double we = wn * e;
             geo_accumulate<is_feature(KIND)>(A, we, f, dg.dR, dg.dZ, dg.dth);
 #pragma unroll
             for (int q = 0; q < NP; ++q) A[G + q] += we * dg.dp[q];
             const double wnn = wn * dg.n;
-            A[G + NP] += wnn * (oma * nd.B);                  // emissivity
-            A[G + NP + 1] += wnn * (scat_unit - eps * nd.B);  // albedo
+            if constexpr (JAC) {
+              Ib[J - S] += (wk * nd.B) * dg.n;
+              Is[J - S] += (wk * scat_unit) * dg.n;
+            } else {
+              A[G + NP] += wnn * (oma * nd.B);                  // emissivity
+              A[G + NP + 1] += wnn * (scat_unit - eps * nd.B);  // albedo
+            }
             // (1 - a) eps, packed with the same rounding as oma * eps
             sw += wnn * P[ZODI_P_SCALE];
             // F, C1..C3 enter through a Phi / R^2, which is 0 when a == 0
@@ -208,7 +214,7 @@ __device__ __forceinline__ void adjoint_group(Grp<Ks...>, std::integer_sequence<
             }
           } else {
             // f = calibration B(T_c) n   (zodipy/brightness.py:59-83)
-            if constexpr (JAC) Iv[J - S] += (wk * nd.B) * dg.n;
+            if constexpr (JAC) Ib[J - S] += (wk * nd.B) * dg.n;
             const double wc = wn * cx.calibration;
             const double wcb = wc * nd.B;
             geo_accumulate<is_feature(KIND)>(A, wcb, f, dg.dR, dg.dZ, dg.dth);
@@ -217,7 +223,7 @@ __device__ __forceinline__ void adjoint_group(Grp<Ks...>, std::integer_sequence<
             const double wt = wc * dg.n * dBdT;
             A[G + NP] += wt * dT_dT0;         // T_0 of this component
             A[G + NP + 1] += wt * dT_ddelta;  // delta of this component
-            sh[0] += wn * nd.B * dg.n;        // calibration
+            if constexpr (!JAC) sh[0] += wn * nd.B * dg.n;  // calibration
           }
         }(),
         ...);
@@ -235,10 +241,22 @@ __device__ __forceinline__ void adjoint_group(Grp<Ks...>, std::integer_sequence<
         [&] {
           constexpr int NP = kind_nparams<kinds[J]>();
           constexpr int G = geo_moments<is_feature(kinds[J])>();
-          const auto& A = std::get<J - S>(acc).v;
+          auto& A = std::get<J - S>(acc).v;
           const int c = first + J;
-          total = __dadd_rn(total, finish<INTEGRAND>(M->comp[c], M->comp_flags[c], Iv[J - S], h,
-                                                     cx.calibration));
+          const ParamPtr P = M->comp[c];
+          const uint32_t flags = M->comp_flags[c];
+          const double hb = y.active ? h * Ib[J - S] : 0.0;
+          double node_sum = Ib[J - S];  // the forward's sum: w B n, or w e n with scattering
+          if constexpr (INTEGRAND == ZODI_INTEGRAND_KELSALL) {
+            const double hs = y.active ? h * Is[J - S] : 0.0;
+            A[G + NP] = P[ZODI_P_ONE_MINUS_ALBEDO] * hb;      // emissivity
+            A[G + NP + 1] = hs - P[ZODI_P_EMISSIVITY] * hb;  // albedo
+            if (flags & ZODI_CF_SCATTER)
+              node_sum = P[ZODI_P_SCALE] * Ib[J - S] + P[ZODI_P_ALBEDO] * Is[J - S];
+          } else {
+            sh[0] += hb;  // calibration
+          }
+          total = __dadd_rn(total, finish<INTEGRAND>(P, flags, node_sum, h, cx.calibration));
           double geo[5];
           geo_finish<is_feature(kinds[J])>(M->comp[c], A, et[J - S], geo);
           if (y.active) {
