@@ -159,7 +159,7 @@ __device__ __forceinline__ void adjoint_group(Grp<Ks...>, std::integer_sequence<
     nd.X2 = s_los * y.uz + y.oz;
     norm_and_inverse_node(nd.X0 * nd.X0 + nd.X1 * nd.X1 + nd.X2 * nd.X2, nd.R, nd.inv_R);
     const double logR = log_pos(nd.R);
-    const double T = T0 * exp_fast(mdelta * logR);
+    const double T = T0 * exp_fast_bounded(mdelta * logR);
     double dBdT;
     nd.B = planck_lookup_slope(T, cx.tb.tab, cx.tb.n_t, cx.tb.T_first, cx.tb.T_last,
                                cx.tb.inv_step, dBdT);

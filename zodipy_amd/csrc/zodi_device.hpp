@@ -56,9 +56,9 @@ __device__ __forceinline__ double pow_pos(double x, double y) {
   return y == 0.0 ? 1.0 : exp_fast(y * log_fast(x));
 }
 
-// x^y for a strictly positive finite base (radii).
+// x^y for a strictly positive finite base (radii): |y log x| <= 745 |y|.
 __device__ __forceinline__ double pow_radius(double x, double y) {
-  return y == 0.0 ? 1.0 : exp_fast(y * log_pos(x));
+  return y == 0.0 ? 1.0 : exp_fast_bounded(y * log_pos(x));
 }
 
 // |X| and 1/|X| from |X|^2: a hardware reciprocal square root refined by two
@@ -208,7 +208,8 @@ __device__ __forceinline__ double cloud_density(ParamPtr P, const CompFrame& f) 
   const double inv_2mu = P[13], half_mu = P[14];
   const double zeta = fabs(f.zc) * f.inv_r;
   const double g = zeta < mu ? (zeta * zeta) * inv_2mu : zeta - half_mu;
-  return n0 * exp_fast(-alpha * log_pos(f.r) - beta * pow_pos(g, gamma));
+  // |log R| <= 745 and g^gamma <= 1 (zeta <= 1): a bounded exponent
+  return n0 * exp_fast_bounded(-alpha * log_pos(f.r) - beta * pow_pos(g, gamma));
 }
 
 __device__ __forceinline__ double pow20(double x) {
@@ -224,7 +225,7 @@ __device__ __forceinline__ double pow20(double x) {
 // branch skips the exponential for the waves whose points all lie that far out.
 __device__ __forceinline__ double band_cutoff_exp(double x) {
   double ex = 0.0;
-  if (x < 38.0) ex = exp_fast(-x);
+  if (x < 38.0) ex = exp_fast_bounded(-x);
   return ex;
 }
 
@@ -248,7 +249,7 @@ __device__ __forceinline__ double band_density(ParamPtr P, uint32_t flags, const
 __device__ __forceinline__ double ring_density(ParamPtr P, const CompFrame& f) {
   const double n0 = P[6], Rr = P[7];
   const double d = f.r - Rr;
-  return n0 * exp_fast(-(d * d) * P[13] - fabs(f.zc) * P[14]);
+  return n0 * exp_fast_bounded(-(d * d) * P[13] - fabs(f.zc) * P[14]);
 }
 
 // The feature's centre line for one ray: (cos psi, sin psi) with
@@ -304,7 +305,7 @@ __device__ __forceinline__ double feature_density(ParamPtr P, const CompFrame& f
   double e = (d * d) * P[13];
   e += fabs(f.zc) * P[14];
   e += (dth * dth) * P[15];
-  return n0 * exp_fast(-e);
+  return n0 * exp_fast_bounded(-e);
 }
 
 // Fan, zodipy/number_density.py:184-218 (zero outside R <= R_outer).

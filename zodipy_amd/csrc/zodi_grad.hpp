@@ -65,7 +65,7 @@ __device__ __forceinline__ DensGrad<5> cloud_grad(ParamPtr P, uint32_t flags, co
   const double logg = log_fast(g);
   const double G = gamma == 0.0 ? 1.0 : exp_fast(gamma * logg);  // g^gamma
   DensGrad<5> o;
-  const double E = exp_fast(-alpha * logr - beta * G);  // n / n_0
+  const double E = exp_fast_bounded(-alpha * logr - beta * G);  // n / n_0
   o.n = n0 * E;
   const double dg_dzeta = inner ? zeta * (2.0 * inv_2mu) : 1.0;
   // gamma g^(gamma-1) written as gamma G / g; its limit at g = 0 is 0 for gamma > 1/2.
@@ -135,7 +135,7 @@ __device__ __forceinline__ DensGrad<4> ring_grad(ParamPtr P, const CompFrame& f)
   const double d = f.r - Rr;
   const double za = fabs(f.zc);
   DensGrad<4> o;
-  const double E = exp_fast(-(d * d) * inv_sr2 - za * inv_sz);
+  const double E = exp_fast_bounded(-(d * d) * inv_sr2 - za * inv_sz);
   o.n = n0 * E;
   o.dR = -2.0 * o.n * d * inv_sr2;
   o.dZ = -o.n * sign_of(f.zc) * inv_sz;
@@ -159,7 +159,7 @@ __device__ __forceinline__ DensGrad<6> feature_grad(ParamPtr P, const CompFrame&
   e += za * inv_sz;
   e += (dth * dth) * inv_sth2;
   DensGrad<6> o;
-  const double E = exp_fast(-e);
+  const double E = exp_fast_bounded(-e);
   o.n = n0 * E;
   o.dR = -2.0 * o.n * d * inv_sr2;
   o.dZ = -o.n * sign_of(f.zc) * inv_sz;

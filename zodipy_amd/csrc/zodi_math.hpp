@@ -43,20 +43,18 @@ __device__ __forceinline__ void stage_math_tables() {
   __syncthreads();
 }
 
-__device__ __forceinline__ double exp_fast(double x) {
+// exp for arguments above -9e15 (or NaN). Below -746 the result rounds to 0 by
+// itself: the integer conversion of x 128/ln 2 stays in range, the reduced
+// argument stays small (its rounding error grows like |x| 1e-16) and ldexp
+// underflows. -inf gives NaN. Callers whose arguments are bounded that way (the
+// temperature law, the densities' Gaussian and power-law factors) call it
+// directly; exp_fast clamps first.
+__device__ __forceinline__ double exp_fast_bounded(double x) {
   constexpr double kInvL = 0x1.71547652b82fep+7;        // 128 / ln 2
   constexpr double kLHi = 0x1.62e42fee00000p-8;         // ln2/128, 32 significant bits
   constexpr double kLLo = 0x1.a39ef35793c76p-40;        // ln2/128 - kLHi
-  // Every exponent in the model is below ~50 (the positive ones are small
-  // temperature and power-law exponents). Arguments below -746 (-inf included)
-  // get the high word of -746, which leaves them in (-746.0005, -746]: the integer
-  // conversion stays in range and the result rounds to 0. One select of a 32-bit
-  // word instead of two. NaN passes.
-  const uint64_t xb = __builtin_bit_cast(uint64_t, x);
-  const uint32_t hi = x < -746.0 ? 0xC0875000u : (uint32_t)(xb >> 32);  // -746.0
-  const double xc = __builtin_bit_cast(double, ((uint64_t)hi << 32) | (uint32_t)xb);
-  const double kd = __builtin_rint(xc * kInvL);
-  double r = fma(-kd, kLHi, xc);
+  const double kd = __builtin_rint(x * kInvL);
+  double r = fma(-kd, kLHi, x);
   r = fma(-kd, kLLo, r);
   const int k = (int)kd;
   const int j = k & 127;
@@ -67,6 +65,16 @@ __device__ __forceinline__ double exp_fast(double x) {
   const double q = fma(p * r, r, r);  // expm1(r)
   const double t = g_exp2[j];
   return __builtin_amdgcn_ldexp(fma(t, q, t), m);
+}
+
+__device__ __forceinline__ double exp_fast(double x) {
+  // Every exponent in the model is below ~50 (the positive ones are small
+  // temperature and power-law exponents). Arguments below -746 (-inf included)
+  // get the high word of -746, which leaves them in (-746.0005, -746]: the result
+  // rounds to 0. One select of a 32-bit word instead of two. NaN passes.
+  const uint64_t xb = __builtin_bit_cast(uint64_t, x);
+  const uint32_t hi = x < -746.0 ? 0xC0875000u : (uint32_t)(xb >> 32);  // -746.0
+  return exp_fast_bounded(__builtin_bit_cast(double, ((uint64_t)hi << 32) | (uint32_t)xb));
 }
 
 // log for positive finite (or NaN) arguments: no special-case handling.
