@@ -59,13 +59,18 @@ __device__ __forceinline__ int64_t find_segment(const int64_t* __restrict__ seg,
 }
 
 // Planck table to LDS as (T_j, slope_j, B_j, 0) entries.
+// LDS entry i: (T_i, slope_i, B_i, B_{i+1} - B_i). Entries from the last
+// temperature on to ZODI_N_TEMPS hold (T_last, 0, B_last, 0), so an index
+// clamped to ZODI_N_TEMPS - 2 reads the table's end value for any n_temps.
 __device__ __forceinline__ void load_table(ModelPtr M, double* sTab) {
   const int n = M->n_temps;
-  for (int i = threadIdx.x; i < n; i += blockDim.x) {
-    sTab[4 * i] = M->tab_T[i];
-    sTab[4 * i + 1] = M->tab_slope[i];
-    sTab[4 * i + 2] = M->tab_B[i];
-    sTab[4 * i + 3] = 0.0;
+  for (int i = threadIdx.x; i < ZODI_N_TEMPS; i += blockDim.x) {
+    const bool in = i < n - 1;
+    const int k = in ? i : n - 1;
+    sTab[4 * i] = M->tab_T[k];
+    sTab[4 * i + 1] = in ? M->tab_slope[k] : 0.0;
+    sTab[4 * i + 2] = M->tab_B[k];
+    sTab[4 * i + 3] = in ? M->tab_B[k + 1] - M->tab_B[k] : 0.0;
   }
   __syncthreads();
 }
@@ -131,9 +136,12 @@ struct Tables {
   int n_t;
 };
 
+// us = T_0 / dT and uo = T_first / dT place T = T_0 R^-delta on the table's
+// index axis: u = R^-delta us - uo.
 template <bool NANFIX = true>
 __device__ __forceinline__ Node node_geometry(const Ray& y, double h, double mid, double xk,
-                                              double T0, double mdelta, const Tables& tb) {
+                                              double us, double uo, double mdelta,
+                                              const Tables& tb) {
   Node nd;
   const double s_los = h * xk + mid;
   nd.X0 = s_los * y.ux + y.ox;
@@ -141,8 +149,8 @@ __device__ __forceinline__ Node node_geometry(const Ray& y, double h, double mid
   nd.X2 = s_los * y.uz + y.oz;
   norm_and_inverse_node(nd.X0 * nd.X0 + nd.X1 * nd.X1 + nd.X2 * nd.X2, nd.R, nd.inv_R);
   // T = T0 R^-delta (zodipy/blackbody.py:16-30), B = interp(T) (zodipy/brightness.py:48)
-  const double T = T0 * pow_radius(nd.R, mdelta);
-  nd.B = planck_lookup<NANFIX>(T, tb.tab, tb.n_t, tb.T_first, tb.T_last, tb.inv_step);
+  const double u = fma(pow_radius(nd.R, mdelta), us, -uo);
+  nd.B = planck_lookup_u<NANFIX>(u, tb.tab);
   nd.phi = 0.0;
   return nd;
 }
@@ -293,6 +301,8 @@ __device__ __forceinline__ void group_pass(Grp<Ks...>, std::integer_sequence<int
   group_range(M, g, y, h, mid);
   const double T0 = M->group_T0[g];
   const double mdelta = -M->group_delta[g];
+  const double us = T0 * cx.tb.inv_step;
+  const double uo = cx.tb.T_first * cx.tb.inv_step;
 
   // The feature's centre line, once per ray.
   FeatureDir fdir[C];
@@ -312,7 +322,7 @@ __device__ __forceinline__ void group_pass(Grp<Ks...>, std::integer_sequence<int
     // Only the interstellar density does not depend on the position (and so on a
     // NaN temperature's cause); without it the lookup skips its NaN pass-through.
     constexpr bool kConstDensity = ((kinds[J] == ZODI_INTERSTELLAR) || ...);
-    Node nd = node_geometry<kConstDensity>(y, h, mid, xk, T0, mdelta, cx.tb);
+    Node nd = node_geometry<kConstDensity>(y, h, mid, xk, us, uo, mdelta, cx.tb);
     if constexpr (INTEGRAND == ZODI_INTEGRAND_KELSALL) {
       if (scatter) {
         nd.phi = phase_function(h * xk + mid, y.ux, y.uy, y.uz, nd.X0, nd.X1, nd.X2, nd.R,

@@ -157,6 +157,26 @@ __device__ __forceinline__ double sphere_distance(const RayGeom& g, double cutof
 // it one interval off (T within ~1e-14 relative of a grid temperature), the
 // neighbouring segment is evaluated at a point it meets within rounding, so the
 // value agrees to ~1e-15. Each LDS entry is (T_j, slope_j, B_j, 0).
+// The same interpolation on the table's index axis, for the forward kernels:
+// u = (T - T_first)/dT, j = floor(u) within [0, ZODI_N_TEMPS - 2] and
+// B = B_j + f (B_{j+1} - B_j) with f = u - j, u first clamped to
+// [0, ZODI_N_TEMPS - 1] (np.interp's end values outside the table; the LDS
+// entries past the last temperature repeat it with a zero difference). The
+// clamps are against constants, so no operand needs canonicalizing, and the
+// temperature itself is never formed. It agrees with planck_lookup to ~1e-14
+// relative (the grid is uniform, T_j = T_first + j dT up to rounding).
+template <bool NANFIX = true>
+__device__ __forceinline__ double planck_lookup_u(double u, const double* __restrict__ tab) {
+  const double uc = fmin(fmax(u, 0.0), (double)(ZODI_N_TEMPS - 1));
+  int j = (int)uc;
+  j = j > ZODI_N_TEMPS - 2 ? ZODI_N_TEMPS - 2 : j;
+  const double f = uc - (double)j;
+  const double* e = tab + 4 * j;
+  const double B = fma(f, e[3], e[2]);
+  if constexpr (NANFIX) return u == u ? B : u;
+  return B;
+}
+
 // NANFIX passes a NaN temperature through (np.interp returns NaN for it). A NaN
 // T comes from a NaN position, which also makes every density that depends on
 // the position NaN: callers whose densities all do may leave the check out.

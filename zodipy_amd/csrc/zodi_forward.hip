@@ -164,6 +164,8 @@ __device__ __forceinline__ void generic_groups(const Ctx& cx, const FwdArgs& a, 
     group_range(M, g, y, h, mid);
     const double T0 = M->group_T0[g];
     const double mdelta = -M->group_delta[g];
+    const double us = T0 * cx.tb.inv_step;
+    const double uo = cx.tb.T_first * cx.tb.inv_step;
     bool scatter = false;
     for (int j = 0; j < cnt; ++j) {
       sAcc[j][tid] = 0.0;
@@ -172,7 +174,7 @@ __device__ __forceinline__ void generic_groups(const Ctx& cx, const FwdArgs& a, 
     for (int k = cx.sub; k < cx.K; k += L) {
       const double xk = M->gl_x[k];
       const double wk = M->gl_w[k];
-      Node nd = node_geometry(y, h, mid, xk, T0, mdelta, cx.tb);
+      Node nd = node_geometry(y, h, mid, xk, us, uo, mdelta, cx.tb);
       if (INTEGRAND == ZODI_INTEGRAND_KELSALL && scatter) {
         nd.phi = phase_function(h * xk + mid, y.ux, y.uy, y.uz, nd.X0, nd.X1, nd.X2, nd.R,
                                 M->shared);
