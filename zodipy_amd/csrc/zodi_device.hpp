@@ -94,28 +94,6 @@ __device__ __forceinline__ void norm_and_inverse_node(double a, double& r, doubl
   r = fma(t, e, t);
 }
 
-// Python/NumPy float remainder a % b for b > 0 (numpy npy_divmod): the exact
-// fmod(a, b), then negative results shifted by +b (one rounding) and zero made +0.
-// For |a| < 4b fmod is formed without a loop: subtracting 2b from |a| in [2b, 4b)
-// and b from |a| in [b, 2b) are both exact (Sterbenz), so the result equals fmod's.
-__device__ __forceinline__ double floor_mod_pos(double a, double b) {
-  double m;
-  if (fabs(a) < 4.0 * b) {
-    m = a;
-    const double two_b = 2.0 * b;
-    if (fabs(m) >= two_b) m = m > 0.0 ? m - two_b : m + two_b;
-    if (fabs(m) >= b) m = m > 0.0 ? m - b : m + b;
-  } else {
-    m = fmod(a, b);
-  }
-  if (m < 0.0) {
-    m += b;
-  } else if (m == 0.0) {
-    m = 0.0;
-  }
-  return m;
-}
-
 // Per-ray quantities shared by every cutoff sphere: the linear coefficient b of
 // the ray/sphere quadratic and the observer's heliocentric distance.
 // zodipy/line_of_sight.py:70-80: b = 2 (x0 cos(lat) cos(lon) + y0 cos(lat) sin(lon)),
@@ -151,20 +129,18 @@ __device__ __forceinline__ double sphere_distance(const RayGeom& g, double cutof
   return np_maximum(q, c / q);
 }
 
-// Linear interpolation in the Planck table with np.interp semantics (clamped to
-// the end values outside [T_0, T_last], NaN in -> NaN out), without branches.
-// The interval index comes from the uniform grid spacing; where rounding puts
-// it one interval off (T within ~1e-14 relative of a grid temperature), the
-// neighbouring segment is evaluated at a point it meets within rounding, so the
-// value agrees to ~1e-15. Each LDS entry is (T_j, slope_j, B_j, 0).
-// The same interpolation on the table's index axis, for the forward kernels:
-// u = (T - T_first)/dT, j = floor(u) within [0, ZODI_N_TEMPS - 2] and
-// B = B_j + f (B_{j+1} - B_j) with f = u - j, u first clamped to
-// [0, ZODI_N_TEMPS - 1] (np.interp's end values outside the table; the LDS
-// entries past the last temperature repeat it with a zero difference). The
-// clamps are against constants, so no operand needs canonicalizing, and the
-// temperature itself is never formed. It agrees with planck_lookup to ~1e-14
-// relative (the grid is uniform, T_j = T_first + j dT up to rounding).
+// Linear interpolation in the Planck table with np.interp semantics, on the
+// table's index axis (the forward kernels): u = (T - T_first)/dT, j = floor(u)
+// within [0, ZODI_N_TEMPS - 2] and B = B_j + f (B_{j+1} - B_j) with f = u - j,
+// u first clamped to [0, ZODI_N_TEMPS - 1] (np.interp's end values outside the
+// table; the LDS entries past the last temperature repeat it with a zero
+// difference). The clamps are against constants, so no operand needs
+// canonicalizing, and the temperature itself is never formed. The grid is
+// uniform (T_j = T_first + j dT up to rounding), so this is np.interp's
+// interpolation to ~1e-14 relative. NANFIX passes a NaN argument through
+// (np.interp returns NaN for a NaN temperature); a NaN T comes from a NaN
+// position, which also makes every density that depends on the position NaN, so
+// callers whose densities all do may leave the check out.
 template <bool NANFIX = true>
 __device__ __forceinline__ double planck_lookup_u(double u, const double* __restrict__ tab) {
   const double uc = fmin(fmax(u, 0.0), (double)(ZODI_N_TEMPS - 1));
@@ -174,21 +150,6 @@ __device__ __forceinline__ double planck_lookup_u(double u, const double* __rest
   const double* e = tab + 4 * j;
   const double B = fma(f, e[3], e[2]);
   if constexpr (NANFIX) return u == u ? B : u;
-  return B;
-}
-
-// NANFIX passes a NaN temperature through (np.interp returns NaN for it). A NaN
-// T comes from a NaN position, which also makes every density that depends on
-// the position NaN: callers whose densities all do may leave the check out.
-template <bool NANFIX = true>
-__device__ __forceinline__ double planck_lookup(double T, const double* __restrict__ tab, int n,
-                                                double T_first, double T_last, double inv_step) {
-  const double t = fmin(fmax(T, T_first), T_last);
-  int j = (int)((t - T_first) * inv_step);
-  j = j > n - 2 ? n - 2 : j;
-  const double* e = tab + 4 * j;
-  const double B = e[1] * (t - e[0]) + e[2];
-  if constexpr (NANFIX) return T == T ? B : T;
   return B;
 }
 
