@@ -65,17 +65,20 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
-// B(T) of the Planck table and its slope dB/dT (0 where the table clamps).
-__device__ __forceinline__ double planck_lookup_slope(double T, const double* __restrict__ tab,
-                                                      int n, double T_first, double T_last,
-                                                      double inv_step, double& dBdT) {
-  const double t = fmin(fmax(T, T_first), T_last);
-  int j = (int)((t - T_first) * inv_step);
-  j = j > n - 2 ? n - 2 : j;
+// B(T) of the Planck table on its index axis, u = (T - T_first)/dT, as
+// planck_lookup_u forms it, and the table slope dB/dT of the interval (0 where
+// the table clamps, u <= 0 or u >= ZODI_N_TEMPS - 1; the LDS entries past the
+// last temperature carry slope 0). NaN passes.
+__device__ __forceinline__ double planck_lookup_u_slope(double u, const double* __restrict__ tab,
+                                                        double& dBdT) {
+  const double uc = fmin(fmax(u, 0.0), (double)(ZODI_N_TEMPS - 1));
+  int j = (int)uc;
+  j = j > ZODI_N_TEMPS - 2 ? ZODI_N_TEMPS - 2 : j;
+  const double f = uc - (double)j;
   const double* e = tab + 4 * j;
-  dBdT = (T > T_first && T < T_last) ? e[1] : 0.0;
-  const double B = e[1] * (t - e[0]) + e[2];
-  return T == T ? B : T;
+  dBdT = (u > 0.0 && u < (double)(ZODI_N_TEMPS - 1)) ? e[1] : 0.0;
+  const double B = fma(f, e[3], e[2]);
+  return u == u ? B : u;
 }
 
 // Register accumulators of one component: the geometry moments (geo_moments), the
@@ -111,7 +114,8 @@ __device__ __forceinline__ void adjoint_group(Grp<Ks...>, std::integer_sequence<
   group_range(M, g, y, h, mid);
   const double T0 = M->group_T0[g];
   const double mdelta = -M->group_delta[g];
-  const double inv_T0 = 1.0 / T0;
+  const double us = T0 * cx.tb.inv_step;  // T = T0 R^-delta on the table's index axis
+  const double uo = cx.tb.T_first * cx.tb.inv_step;
 
   double W[N];  // h times the weight of each component
   if constexpr (JAC) {
@@ -159,12 +163,11 @@ __device__ __forceinline__ void adjoint_group(Grp<Ks...>, std::integer_sequence<
     nd.X2 = s_los * y.uz + y.oz;
     norm_and_inverse_node(nd.X0 * nd.X0 + nd.X1 * nd.X1 + nd.X2 * nd.X2, nd.R, nd.inv_R);
     const double logR = log_pos(nd.R);
-    const double T = T0 * exp_fast_bounded(mdelta * logR);
+    const double Rd = exp_fast_bounded(mdelta * logR);  // R^-delta = dT/dT0
     double dBdT;
-    nd.B = planck_lookup_slope(T, cx.tb.tab, cx.tb.n_t, cx.tb.T_first, cx.tb.T_last,
-                               cx.tb.inv_step, dBdT);
-    const double dT_dT0 = T * inv_T0;
-    const double dT_ddelta = -T * logR;
+    nd.B = planck_lookup_u_slope(fma(Rd, us, -uo), cx.tb.tab, dBdT);
+    const double dT_dT0 = Rd;
+    const double dT_ddelta = -(T0 * Rd) * logR;
     PhaseGrad pg = {need_phase ? phi_const : 0.0, 0.0, 0.0, 0.0};
     if (need_angle) {
       pg = phase_grad(y.ux, y.uy, y.uz, nd.X0, nd.X1, nd.X2, nd.inv_R, Mk->shared);
