@@ -53,7 +53,7 @@ __device__ __forceinline__ double np_maximum(double a, double b) {
 // exp_fast(y log x). The result differs from a correctly rounded pow by about
 // |y log x| ulp; x = 0 gives 0 for y > 0; y = 0 gives 1 like pow.
 __device__ __forceinline__ double pow_pos(double x, double y) {
-  return y == 0.0 ? 1.0 : exp_fast(y * log_fast(x));
+  return y == 0.0 ? 1.0 : exp_fast(y * log_nonneg(x));
 }
 
 // x^y for a strictly positive finite base (radii): |y log x| <= 745 |y|.

@@ -62,7 +62,7 @@ __device__ __forceinline__ DensGrad<5> cloud_grad(ParamPtr P, uint32_t flags, co
   const bool inner = zeta < mu;
   const double g = inner ? (zeta * zeta) * inv_2mu : zeta - half_mu;
   const double logr = (flags & ZODI_CF_CENTERED) ? logR : log_pos(f.r);
-  const double logg = log_fast(g);
+  const double logg = log_nonneg(g);
   const double G = gamma == 0.0 ? 1.0 : exp_fast(gamma * logg);  // g^gamma
   DensGrad<5> o;
   const double E = exp_fast_bounded(-alpha * logr - beta * G);  // n / n_0
@@ -95,7 +95,7 @@ __device__ __forceinline__ DensGrad<5> band_grad(ParamPtr P, uint32_t flags, con
   const double zd6 = zd4 * zd2;
   // zd^p = exp(p log zd) needs log 0 = -inf; with p = 4 the log only enters the
   // p partial, which zd = 0 masks, so the special cases can be left out.
-  const double logzd = (flags & ZODI_CF_P_IS_4) ? log_pos(zd) : log_fast(zd);
+  const double logzd = (flags & ZODI_CF_P_IS_4) ? log_pos(zd) : log_nonneg(zd);
   const double zdp = (flags & ZODI_CF_P_IS_4) ? zd4 : (p == 0.0 ? 1.0 : exp_fast(p * logzd));
   const double t1 = three_n0 * f.inv_r;
   const double t2 = exp_fast(-zd6);
@@ -210,7 +210,7 @@ __device__ __forceinline__ LatTerm lat_term(const CompFrame& f, double inv_Z0, d
   t.u = pow_pos(L, eps);
   const bool pos = L > 0.0;
   t.du_db = pos ? eps * t.u * rcp_fast(L) * sign_of(t.beta) : 0.0;
-  t.du_deps = pos ? t.u * log_fast(L) : 0.0;
+  t.du_deps = pos ? t.u * log_pos(L) : 0.0;
   t.deps_dz = t.inner ? -sign_of(f.zc) * inv_Z0 : 0.0;
   return t;
 }
@@ -235,7 +235,7 @@ __device__ __forceinline__ DensGrad<5> fan_grad(ParamPtr P, const CompFrame& f) 
   o.dth = 0.0;
   o.dp[0] = -o.n * log_pos(f.r);
   o.dp[1] = t.inner ? o.n * dl_deps * t.za * (P[13] * P[13]) : 0.0;
-  o.dp[2] = o.n * log_fast(t.cb);
+  o.dp[2] = o.n * log_nonneg(t.cb);
   o.dp[3] = -o.n * su;
   o.dp[4] = 0.0;
   return o;

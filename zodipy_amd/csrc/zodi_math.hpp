@@ -96,12 +96,10 @@ __device__ __forceinline__ double log_pos(double x) {
   return fma(de, kLn2Hi, g_log_c[j] + fma(de, kLn2Lo, l1p));
 }
 
-// log for any argument: 0 -> -inf, negative or NaN -> NaN, +inf -> +inf.
-__device__ __forceinline__ double log_fast(double x) {
+// log for finite non-negative (or NaN) arguments: 0 -> -inf.
+__device__ __forceinline__ double log_nonneg(double x) {
   const double res = log_pos(x);
-  if (x > 0.0 && x < __builtin_inf()) return res;
-  if (x == 0.0) return -__builtin_inf();
-  return x == __builtin_inf() ? x : __builtin_nan("");
+  return x == 0.0 ? -__builtin_inf() : res;
 }
 
 }  // namespace zodi
