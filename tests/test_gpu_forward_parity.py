@@ -258,6 +258,44 @@ def test_feature_longitude_axis_and_wrap(gpu):
         assert_parity(got, want, rtol=1e-12)
 
 
+@pytest.mark.parametrize("name", ["dirbe", "rrm-experimental"])
+def test_short_planck_table(gpu, name):
+    """A model block with fewer than 100 tabulated temperatures (the C ABI allows
+    2..100): np.interp holds B at the last one above it. With the first 40 of the
+    registry table (40 - 240.9 K; an observer at 1 AU sees up to ~290 K) and
+    NaN in every unused entry, the forward must equal that of the full block
+    whose values beyond the 40th temperature repeat B_39 (the same piecewise-
+    linear function), and read nothing past the 40th entry."""
+    import dataclasses
+
+    from zodipy_amd.runtime import DeviceModel
+
+    model = Model(25 * u.micron, name=name)
+    packed = model.device_model.packed
+    n = 40
+    short = packed.block.copy()
+    short["n_temps"] = n
+    short["tab_T"][0, n:] = np.nan
+    short["tab_B"][0, n:] = np.nan
+    short["tab_slope"][0, n - 1:] = np.nan
+    t = short["tab_T"][0]
+    short["tab_inv_step"] = (n - 1) / (t[n - 1] - t[0])
+    flat = packed.block.copy()
+    flat["tab_B"][0, n:] = flat["tab_B"][0, n - 1]
+    flat["tab_slope"][0, n - 1:] = 0.0
+    v = torch.as_tensor(healpix.pix2vec(8), device="cuda")
+    obs = np.array([0.95, 0.3, 0.01])
+    out = []
+    for block in (short, flat):
+        dm = DeviceModel(dataclasses.replace(packed, block=block))
+        out.append(dm.forward(v, obs, obs).cpu().numpy())
+    assert np.all(np.isfinite(out[0]))
+    np.testing.assert_allclose(out[0], out[1], rtol=1e-14, atol=0)
+    # some ray-node temperatures lie above the short table's last one
+    assert not np.allclose(out[0], model.evaluate_vectors(
+        v.cpu().numpy(), obspos=obs, earthpos=obs, return_comps=True).value, rtol=1e-6)
+
+
 def test_device_tensor_output_and_stream(gpu):
     v = torch.as_tensor(random_dirs(4096, seed=30), device="cuda")
     obs = np.array([1.0, 0.0, 0.0])
