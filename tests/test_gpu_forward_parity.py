@@ -80,7 +80,7 @@ def test_registry_models_off_ecliptic_observer(gpu, name, x, kw):
 
 
 def test_bandpass_gaussian_25um(gpu):
-    """Config-2 style bandpass in microns (normalised Gaussian weights)."""
+    """Gaussian bandpass of 50 wavelengths around 25 um (normalised weights)."""
     x = np.linspace(17.5, 32.5, 50) * u.micron
     w = np.exp(-0.5 * ((x.value - 25) / 2.5) ** 2)
     v = healpix.pix2vec(8)
@@ -244,7 +244,10 @@ def test_feature_longitude_axis_and_wrap(gpu):
                            np.stack([-np.zeros(7), np.zeros(7), z]),
                            np.stack([np.zeros(7), -np.zeros(7), z]),
                            np.stack([-np.zeros(7), -np.zeros(7), z])], axis=1)
-    for earth in ([1.0, 0.0, 0.0], [-1.0, 0.0, 0.0], [-1.0, -0.0, 0.0], [0.3, -0.95, 0.0]):
+    # the last three put the Earth on the feature's axis (E' = +-0): atan2 of
+    # signed zeros gives its longitude (0 or pi)
+    for earth in ([1.0, 0.0, 0.0], [-1.0, 0.0, 0.0], [-1.0, -0.0, 0.0], [0.3, -0.95, 0.0],
+                  [0.0, 0.0, 0.0], [-0.0, 0.0, 0.1], [0.0, -0.0, -0.1]):
         earth = np.array(earth)
         lon_e = np.arctan2(earth[1], earth[0])
         theta_f = np.radians(-10.0)  # the DIRBE feature's offset
