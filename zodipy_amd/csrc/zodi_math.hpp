@@ -1,10 +1,11 @@
 // Table-driven FP64 exp and log for the zodiacal kernels (gfx950).
 //
-// exp: Tang's table-driven method (ACM TOMS 15(2):144-157, 1989). x = (128 m + j) ln2/128 + r,
-//   |r| <= ln2/256; exp(x) = 2^m 2^(j/128) (1 + expm1(r)) with expm1(r) from its Taylor
-//   series to r^5 (truncation < 6e-19). Within 1 ulp of the exact value (checked
-//   against mpmath on 5.5e5 arguments); exactly rounded 1 + expm1(r) for |x| < ln2/256,
-//   which keeps 1 - exp(-x) faithful for small x.
+// exp: Tang's table-driven method (ACM TOMS 15(2):144-157, 1989). x = (256 m + j) ln2/256 + r,
+//   |r| <= ln2/512; exp(x) = 2^m 2^(j/256) (1 + expm1(r)) with expm1(r) from its Taylor
+//   series to r^4 (truncation < 4e-17), evaluated as r + r^2 (1/2 + r (r + 4)/24): the
+//   4 is an inline constant of the FMA unit, so no step reads two scalar constants.
+//   Within 1 ulp of the exact value (checked against mpmath on 6e5 arguments); exactly
+//   rounded 1 + expm1(r) for |x| < ln2/512, which keeps 1 - exp(-x) faithful for small x.
 // log: Tang's table-driven method (ACM TOMS 16(4):378-400, 1990). x = 2^e m, m in [1/2, 1),
 //   c_j = 1/2 + j/512 with j the top 8 mantissa bits, r = (m - c_j)/c_j in [0, 1/256);
 //   log(x) = e ln2 + log(c_j) + log1p(r) with log1p from its Taylor series to r^6
@@ -23,45 +24,42 @@
 
 namespace zodi {
 
-__constant__ double kMathTablesConst[128 + 256 + 256] = {ZODI_EXP2_TABLE, ZODI_LOG_INV_TABLE,
+__constant__ double kMathTablesConst[256 + 256 + 256] = {ZODI_EXP2_TABLE, ZODI_LOG_INV_TABLE,
                                                          ZODI_LOG_C_TABLE};
-// Taylor coefficients: expm1 r^5..r^3 (1/120, 1/24, 1/6); log1p r^6..r^3.
-__constant__ double kExpPoly[3] = {1.0 / 120.0, 1.0 / 24.0, 1.0 / 6.0};
+// Taylor coefficients of log1p, r^6..r^3.
 __constant__ double kLogPoly[4] = {-1.0 / 6.0, 1.0 / 5.0, -1.0 / 4.0, 1.0 / 3.0};
 
-__shared__ double g_exp2[128];
+__shared__ double g_exp2[256];
 __shared__ double g_log_inv[256];
 __shared__ double g_log_c[256];
 
 // Copy the tables to LDS; every thread of the block must call this before use.
 __device__ __forceinline__ void stage_math_tables() {
-  for (int i = threadIdx.x; i < 128; i += blockDim.x) g_exp2[i] = kMathTablesConst[i];
   for (int i = threadIdx.x; i < 256; i += blockDim.x) {
-    g_log_inv[i] = kMathTablesConst[128 + i];
-    g_log_c[i] = kMathTablesConst[384 + i];
+    g_exp2[i] = kMathTablesConst[i];
+    g_log_inv[i] = kMathTablesConst[256 + i];
+    g_log_c[i] = kMathTablesConst[512 + i];
   }
   __syncthreads();
 }
 
 // exp for arguments above -9e15 (or NaN). Below -746 the result rounds to 0 by
-// itself: the integer conversion of x 128/ln 2 stays in range, the reduced
-// argument stays small (its rounding error grows like |x| 1e-16) and ldexp
-// underflows. -inf gives NaN. Callers whose arguments are bounded that way (the
+// itself: the reduced argument stays small (its rounding error grows like
+// |x| 1e-16) and ldexp underflows (below about -5.8e6 the integer conversion of
+// x 256/ln 2 saturates, which still gives 0). -inf gives NaN. Callers whose arguments are bounded that way (the
 // temperature law, the densities' Gaussian and power-law factors) call it
 // directly; exp_fast clamps first.
 __device__ __forceinline__ double exp_fast_bounded(double x) {
-  constexpr double kInvL = 0x1.71547652b82fep+7;        // 128 / ln 2
-  constexpr double kLHi = 0x1.62e42fee00000p-8;         // ln2/128, 32 significant bits
-  constexpr double kLLo = 0x1.a39ef35793c76p-40;        // ln2/128 - kLHi
+  constexpr double kInvL = 0x1.71547652b82fep+8;        // 256 / ln 2
+  constexpr double kLHi = 0x1.62e42fee00000p-9;         // ln2/256, 32 significant bits
+  constexpr double kLLo = 0x1.a39ef35793c76p-41;        // ln2/256 - kLHi
   const double kd = __builtin_rint(x * kInvL);
   double r = fma(-kd, kLHi, x);
   r = fma(-kd, kLLo, r);
   const int k = (int)kd;
-  const int j = k & 127;
-  const int m = k >> 7;
-  double p = fma(kExpPoly[0], r, kExpPoly[1]);
-  p = fma(p, r, kExpPoly[2]);
-  p = fma(p, r, 0.5);
+  const int j = k & 255;
+  const int m = k >> 8;
+  const double p = fma((r + 4.0) * (1.0 / 24.0), r, 0.5);
   const double q = fma(p * r, r, r);  // expm1(r)
   const double t = g_exp2[j];
   return __builtin_amdgcn_ldexp(fma(t, q, t), m);
