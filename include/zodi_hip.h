@@ -313,7 +313,9 @@ enum {
  * the weights g. The two paths agree to rounding. launch may be NULL; otherwise
  * block must be 0 or 256, lanes_per_ray 0 (chosen from n_rays), 1, 2 or 4 (lanes
  * sharing one ray's nodes), blocks_per_cu 0..64 (0 = 2), and variant the
- * components evaluated per pass over a group's nodes: 0 (default, 2), 1 or 2.
+ * components evaluated per pass over a group's nodes: 0 (default: 4 in the
+ * Jacobian pass, 2 otherwise), 1, 2 or 4 (four in the Jacobian pass, the third
+ * and fourth with their sums in LDS; two in the weighted pass).
  * Supported layouts: the Kelsall registry models (cloud and three bands,
  * optionally followed by the ring and the feature) and the RRM model (its
  * registry grouping, or one component per group); others return

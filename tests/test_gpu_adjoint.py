@@ -176,7 +176,8 @@ def test_jacobian_matches_twin_forward_and_vjp(setup):
 
 
 def test_jacobian_leading_dimension_and_chunks(setup):
-    """ld_jac > n_rays leaves the padding untouched; one or two components per pass agree."""
+    """ld_jac > n_rays leaves the padding untouched; one, two and four (the default,
+    two of them summed in LDS) components per pass agree."""
     from zodipy_amd import _lib
     from zodipy_amd._lib import Launch
     from zodipy_amd.runtime import _stream
@@ -193,13 +194,14 @@ def test_jacobian_leading_dimension_and_chunks(setup):
     _lib.check(st, "zodi_jacobian")
     assert torch.equal(jp[:, :n], J) and torch.equal(Ip, I)
     assert torch.all(jp[:, n:] == -7.0)
-    prob.launch = Launch(0, 1, 2, 1)
-    try:
-        I1, J1 = prob.jacobian(theta)
-    finally:
-        prob.launch = None
-    torch.testing.assert_close(I1, I, rtol=1e-14, atol=0)
-    torch.testing.assert_close(J1, J, rtol=1e-12, atol=1e-14 * float(J.abs().max()))
+    for chunk in (1, 2, 4):
+        prob.launch = Launch(0, 1, 2, chunk)
+        try:
+            I1, J1 = prob.jacobian(theta)
+        finally:
+            prob.launch = None
+        torch.testing.assert_close(I1, I, rtol=1e-14, atol=0)
+        torch.testing.assert_close(J1, J, rtol=1e-12, atol=1e-14 * float(J.abs().max()))
 
 
 def test_chi2_paths_agree_on_segments_and_per_ray(gpu):
@@ -377,10 +379,11 @@ def test_unsupported_layout_raises(gpu, cloud_ring_model):
         prob.vjp(prob.space.theta0, torch.ones(v.shape[1], dtype=torch.float64, device="cuda"))
 
 
-@pytest.mark.parametrize("chunk,split", [(1, 1), (1, 2), (1, 4), (2, 1), (2, 2), (2, 4)])
+@pytest.mark.parametrize("chunk,split",
+                         [(1, 1), (1, 2), (1, 4), (2, 1), (2, 2), (2, 4), (4, 1), (4, 2)])
 def test_launch_variants_match_twin(gpu, chunk, split):
-    """Every adjoint kernel variant: components per pass (launch.variant) and lanes
-    per ray (launch.lanes_per_ray)."""
+    """Every adjoint kernel variant: components per pass (launch.variant; the
+    weighted pass runs 4 as 2) and lanes per ray (launch.lanes_per_ray)."""
     from zodipy_amd._lib import Launch
 
     x = 1.25 * u.micron

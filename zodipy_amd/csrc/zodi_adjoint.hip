@@ -81,14 +81,88 @@ __device__ __forceinline__ double planck_lookup_u_slope(double u, const double* 
   return u == u ? B : u;
 }
 
-// Register accumulators of one component: the geometry moments (geo_moments), the
-// kind's parameters (NP) and the two spectral parameters.
+// Accumulators of one component: the geometry moments (geo_moments), the kind's
+// parameters (NP) and two more slots: the spectral parameters (emissivity and
+// albedo, or RRM's T_0 and delta). In the Jacobian pass a Kelsall component
+// sums sum w n B and sum w n F Phi / R^2 in those two slots instead (its
+// spectral partials follow from them after the node loop); an RRM kind (FAN and
+// later) sums sum w n B in a third slot.
+template <int KIND>
+constexpr int acc_slots() {
+  return geo_moments<is_feature(KIND)>() + kind_nparams<KIND>() + 2 + (KIND >= ZODI_FAN ? 1 : 0);
+}
+
 template <int KIND>
 struct CompAcc {
-  static constexpr int G = geo_moments<is_feature(KIND)>();
-  static constexpr int NP = kind_nparams<KIND>();
-  double v[G + NP + 2];
+  static constexpr int kSlots = acc_slots<KIND>();
+  double v[kSlots];
+  __device__ __forceinline__ void add(int q, double x) { v[q] += x; }
+  __device__ __forceinline__ double get(int q) const { return v[q]; }
 };
+
+// The same accumulators in LDS, one slot per 64-lane row (slot q of a lane at
+// p[64 q]): the components of a Jacobian chunk beyond kJacRegComps keep theirs
+// here (in_lds), so one pass over a group's nodes serves four components without
+// spilling registers. Each lane owns its slots, and each sum is one ds_add_f64
+// per node in program order, so the sums repeat bit for bit; they agree with the
+// register sums to rounding (there the product may fuse into the addition).
+template <int KIND>
+struct LdsAcc {
+  static constexpr int kSlots = acc_slots<KIND>();
+  double* p;
+  __device__ __forceinline__ void add(int q, double x) const {
+    __hip_atomic_fetch_add(p + q * kWave, x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+  }
+  __device__ __forceinline__ double get(int q) const { return p[q * kWave]; }
+};
+
+template <int KIND, bool IN_LDS>
+struct AccOf {
+  using type = CompAcc<KIND>;
+};
+template <int KIND>
+struct AccOf<KIND, true> {
+  using type = LdsAcc<KIND>;
+};
+
+// Components per Jacobian chunk whose accumulators stay in registers.
+constexpr int kJacRegComps = 2;
+
+template <bool JAC, int CHUNK>
+constexpr int reg_comps() {
+  return JAC && CHUNK > kJacRegComps ? kJacRegComps : CHUNK;
+}
+
+// Whether component i of a chunk of n, nreg of them in registers, sums in LDS:
+// the first n - nreg. A wave's LDS operations complete in order, so the first
+// table read after a component's ds_add_f64 waits for them; the next component's
+// arithmetic before its first table read covers part of that wait, while after
+// the chunk's last component only the next node's radius precedes its log.
+constexpr bool in_lds(int i, int n, int nreg) { return i < n - nreg; }
+
+// LDS slots per lane for the largest chunk of a layout (0 without LDS components).
+template <int CHUNK, int NREG, int... Ks>
+constexpr int group_lds_slots(Grp<Ks...>) {
+  constexpr int C = sizeof...(Ks);
+  const int slots[C] = {acc_slots<Ks>()...};
+  int best = 0;
+  for (int s = 0; s < C; s += CHUNK) {
+    const int n = C - s < CHUNK ? C - s : CHUNK;
+    int t = 0;
+    for (int i = 0; i < n; ++i) t += in_lds(i, n, NREG) ? slots[s + i] : 0;
+    best = t > best ? t : best;
+  }
+  return best;
+}
+
+template <int CHUNK, int NREG, int INTEGRAND, class... Gs>
+constexpr int layout_lds_slots(Layout<INTEGRAND, Gs...>) {
+  int best = 0;
+  ((best = group_lds_slots<CHUNK, NREG>(Gs{}) > best ? group_lds_slots<CHUNK, NREG>(Gs{})
+                                                      : best),
+   ...);
+  return best;
+}
 
 // Components J (consecutive, from S) of group g, whose first component is `first`.
 // Lane `sub` of the SPLIT lanes sharing the ray takes nodes sub, sub + SPLIT, ...
@@ -99,16 +173,24 @@ struct CompAcc {
 // as a column of the Jacobian, dI_r/dtheta_p -> jac[p][r], instead of reducing
 // them over the wave; `total` receives I_r, each component's node sum finished
 // as the forward kernel finishes it and added in component order.
-template <int INTEGRAND, int SPLIT, bool JAC, int S, int... Ks, int... J>
+template <int INTEGRAND, int SPLIT, bool JAC, int NREG, int S, int... Ks, int... J>
 __device__ __forceinline__ void adjoint_group(Grp<Ks...>, std::integer_sequence<int, J...>,
                                               const Ctx& cx, const AdjArgs& a, const Ray& y,
                                               int g, int first, double gscale, double* row,
                                               int lane, int sub, double (&sh)[6],
-                                              double& total) {
+                                              double& total, double* lane_acc) {
   static_assert(!JAC || SPLIT == 1, "the Jacobian pass uses one lane per ray");
+  static_assert(JAC || NREG >= (int)sizeof...(J), "LDS accumulators are for the Jacobian pass");
   constexpr int C = sizeof...(Ks);
   constexpr int N = sizeof...(J);
   constexpr int kinds[C] = {Ks...};
+  // LDS slots of each chunk component (0 for those in registers) and their offsets.
+  constexpr int lds_len[N] = {(in_lds(J - S, N, NREG) ? acc_slots<kinds[J]>() : 0)...};
+  constexpr auto lds_off = [](const int(&len)[N], int i) {
+    int o = 0;
+    for (int t = 0; t < i; ++t) o += len[t];
+    return o;
+  };
   const ModelPtr M = cx.M;
   double h, mid;
   group_range(M, g, y, h, mid);
@@ -143,13 +225,20 @@ __device__ __forceinline__ void adjoint_group(Grp<Ks...>, std::integer_sequence<
                                          M->shared[ZODI_S_C3] != 0.0);
   const double phi_const = M->shared[ZODI_S_PHASE_NORM] * (M->shared[ZODI_S_C1] + 1.0);
 
-  std::tuple<CompAcc<kinds[J]>...> acc{};
-  // JAC: node sums per component, Ib = sum w n B and (Kelsall) Is = sum w n F Phi / R^2.
+  std::tuple<typename AccOf<kinds[J], in_lds(J - S, N, NREG)>::type...> acc{};
+  (
+      [&] {
+        if constexpr (in_lds(J - S, N, NREG)) {
+          auto& A = std::get<J - S>(acc);
+          A.p = lane_acc + lds_off(lds_len, J - S) * kWave;
+#pragma unroll
+          for (int q = 0; q < A.kSlots; ++q) A.p[q * kWave] = 0.0;
+        }
+      }(),
+      ...);
+  // JAC: node sums per component, sum w n B and (Kelsall) sum w n F Phi / R^2.
   // The component's integral, and its emissivity and albedo partials (and RRM's
   // calibration partial), are linear in them: they are formed after the loop.
-  double Ib[N], Is[N];
-  ((Ib[J - S] = 0.0), ...);
-  ((Is[J - S] = 0.0), ...);
 
   const int K = y.active ? cx.K : 0;
   for (int k = sub; k < K; k += SPLIT) {
@@ -183,7 +272,7 @@ __device__ __forceinline__ void adjoint_group(Grp<Ks...>, std::integer_sequence<
           const uint32_t flags = Mk->comp_flags[first + J];
           const CompFrame f = comp_frame(P, flags, nd.X0, nd.X1, nd.X2, nd.R, nd.inv_R);
           const DensGrad<NP> dg = density_grad<KIND>(P, flags, f, fdir[J - S], logR);
-          auto& A = std::get<J - S>(acc).v;
+          auto& A = std::get<J - S>(acc);
           const double wn = wk * W[J - S];
           if constexpr (INTEGRAND == ZODI_INTEGRAND_KELSALL) {
             // f = [(1 - a) eps B + a F Phi / R^2] n   (zodipy/brightness.py:21-56)
@@ -195,14 +284,14 @@ __device__ __forceinline__ void adjoint_group(Grp<Ks...>, std::integer_sequence<
             const double we = wn * e;
             geo_accumulate<is_feature(KIND)>(A, we, f, dg.dR, dg.dZ, dg.dth);
 #pragma unroll
-            for (int q = 0; q < NP; ++q) A[G + q] += we * dg.dp[q];
+            for (int q = 0; q < NP; ++q) A.add(G + q, we * dg.dp[q]);
             const double wnn = wn * dg.n;
             if constexpr (JAC) {
-              Ib[J - S] += (wk * nd.B) * dg.n;
-              Is[J - S] += (wk * scat_unit) * dg.n;
+              A.add(G + NP, (wk * nd.B) * dg.n);
+              A.add(G + NP + 1, (wk * scat_unit) * dg.n);
             } else {
-              A[G + NP] += wnn * (oma * nd.B);                  // emissivity
-              A[G + NP + 1] += wnn * (scat_unit - eps * nd.B);  // albedo
+              A.add(G + NP, wnn * (oma * nd.B));                  // emissivity
+              A.add(G + NP + 1, wnn * (scat_unit - eps * nd.B));  // albedo
             }
             // (1 - a) eps, packed with the same rounding as oma * eps
             sw += wnn * P[ZODI_P_SCALE];
@@ -217,15 +306,15 @@ __device__ __forceinline__ void adjoint_group(Grp<Ks...>, std::integer_sequence<
             }
           } else {
             // f = calibration B(T_c) n   (zodipy/brightness.py:59-83)
-            if constexpr (JAC) Ib[J - S] += (wk * nd.B) * dg.n;
+            if constexpr (JAC) A.add(G + NP + 2, (wk * nd.B) * dg.n);
             const double wc = wn * cx.calibration;
             const double wcb = wc * nd.B;
             geo_accumulate<is_feature(KIND)>(A, wcb, f, dg.dR, dg.dZ, dg.dth);
 #pragma unroll
-            for (int q = 0; q < NP; ++q) A[G + q] += wcb * dg.dp[q];
+            for (int q = 0; q < NP; ++q) A.add(G + q, wcb * dg.dp[q]);
             const double wt = wc * dg.n * dBdT;
-            A[G + NP] += wt * dT_dT0;         // T_0 of this component
-            A[G + NP + 1] += wt * dT_ddelta;  // delta of this component
+            A.add(G + NP, wt * dT_dT0);         // T_0 of this component
+            A.add(G + NP + 1, wt * dT_ddelta);  // delta of this component
             if constexpr (!JAC) sh[0] += wn * nd.B * dg.n;  // calibration
           }
         }(),
@@ -242,20 +331,30 @@ __device__ __forceinline__ void adjoint_group(Grp<Ks...>, std::integer_sequence<
     // The ray's Jacobian entries of these components (coalesced over the wave).
     (
         [&] {
-          constexpr int NP = kind_nparams<kinds[J]>();
-          constexpr int G = geo_moments<is_feature(kinds[J])>();
-          auto& A = std::get<J - S>(acc).v;
+          constexpr int KIND = kinds[J];
+          constexpr int NP = kind_nparams<KIND>();
+          constexpr int G = geo_moments<is_feature(KIND)>();
+          const auto& AC = std::get<J - S>(acc);
+          double A[acc_slots<KIND>()];
+#pragma unroll
+          for (int q = 0; q < acc_slots<KIND>(); ++q) A[q] = AC.get(q);
           const int c = first + J;
           const ParamPtr P = M->comp[c];
           const uint32_t flags = M->comp_flags[c];
-          const double hb = y.active ? h * Ib[J - S] : 0.0;
-          double node_sum = Ib[J - S];  // the forward's sum: w B n, or w e n with scattering
+          double Ib;  // sum w n B
           if constexpr (INTEGRAND == ZODI_INTEGRAND_KELSALL) {
-            const double hs = y.active ? h * Is[J - S] : 0.0;
+            Ib = A[G + NP];
+          } else {
+            Ib = A[G + NP + 2];
+          }
+          const double hb = y.active ? h * Ib : 0.0;
+          double node_sum = Ib;  // the forward's sum: w B n, or w e n with scattering
+          if constexpr (INTEGRAND == ZODI_INTEGRAND_KELSALL) {
+            const double Is = A[G + NP + 1];  // sum w n F Phi / R^2
+            const double hs = y.active ? h * Is : 0.0;
             A[G + NP] = P[ZODI_P_ONE_MINUS_ALBEDO] * hb;      // emissivity
             A[G + NP + 1] = hs - P[ZODI_P_EMISSIVITY] * hb;  // albedo
-            if (flags & ZODI_CF_SCATTER)
-              node_sum = P[ZODI_P_SCALE] * Ib[J - S] + P[ZODI_P_ALBEDO] * Is[J - S];
+            if (flags & ZODI_CF_SCATTER) node_sum = P[ZODI_P_SCALE] * Ib + P[ZODI_P_ALBEDO] * Is;
           } else {
             sh[0] += hb;  // calibration
           }
@@ -281,7 +380,7 @@ __device__ __forceinline__ void adjoint_group(Grp<Ks...>, std::integer_sequence<
         [&] {
           constexpr int NP = kind_nparams<kinds[J]>();
           constexpr int G = geo_moments<is_feature(kinds[J])>();
-          auto& A = std::get<J - S>(acc).v;
+          auto& A = std::get<J - S>(acc).v;  // registers: NREG covers the chunk
           const int c = first + J;
           double geo[5];
           geo_finish<is_feature(kinds[J])>(M->comp[c], A, et[J - S], geo);
@@ -313,15 +412,15 @@ template <int INTEGRAND, int SPLIT, int CHUNK, bool JAC, int S, int... Ks>
 __device__ __forceinline__ void adjoint_chunks(Grp<Ks...> grp, const Ctx& cx, const AdjArgs& a,
                                                const Ray& y, int g, int first, double gscale,
                                                double* row, int lane, int sub, double (&sh)[6],
-                                               double& total) {
+                                               double& total, double* lane_acc) {
   constexpr int C = sizeof...(Ks);
   if constexpr (S < C) {
     constexpr int N = (C - S) < CHUNK ? (C - S) : CHUNK;
-    adjoint_group<INTEGRAND, SPLIT, JAC, S>(
+    adjoint_group<INTEGRAND, SPLIT, JAC, reg_comps<JAC, CHUNK>(), S>(
         grp, shift_seq<S>(std::make_integer_sequence<int, N>{}), cx, a, y, g, first, gscale, row,
-        lane, sub, sh, total);
+        lane, sub, sh, total, lane_acc);
     adjoint_chunks<INTEGRAND, SPLIT, CHUNK, JAC, S + N>(grp, cx, a, y, g, first, gscale, row,
-                                                        lane, sub, sh, total);
+                                                        lane, sub, sh, total, lane_acc);
   }
 }
 
@@ -331,13 +430,13 @@ __device__ __forceinline__ void adjoint_chunks(Grp<Ks...> grp, const Ctx& cx, co
 template <int SPLIT, int CHUNK, bool JAC, int INTEGRAND, class... Gs>
 __device__ __forceinline__ void run_adjoint(Layout<INTEGRAND, Gs...>, const Ctx& cx,
                                             const AdjArgs& a, const Ray& y, double gscale,
-                                            double* row, int lane, int sub) {
+                                            double* row, int lane, int sub, double* lane_acc) {
   double sh[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   double total = 0.0;
   int g = 0;
   int first = 0;
   ((adjoint_chunks<INTEGRAND, SPLIT, CHUNK, JAC, 0>(Gs{}, cx, a, y, g, first, gscale, row, lane,
-                                                    sub, sh, total),
+                                                    sub, sh, total, lane_acc),
     ++g, first += Gs::size),
    ...);
   const ModelPtr M = cx.M;
@@ -366,6 +465,10 @@ adjoint_layout(AdjArgs a) {
   constexpr int RAYS = kWave / SPLIT;
   extern __shared__ double sRows[];  // kAdjWaves rows of n_params + 1 (not used by JAC)
   __shared__ double sTab[4 * ZODI_N_TEMPS];
+  // JAC chunks of more than kJacRegComps components: LDS accumulators, per wave
+  // kLdsSlots rows of 64 lanes.
+  constexpr int kLdsSlots = layout_lds_slots<CHUNK, reg_comps<JAC, CHUNK>()>(LAYOUT{});
+  __shared__ double sAcc[kLdsSlots > 0 ? kAdjWaves * kWave * kLdsSlots : 1];
   const ModelPtr M = as_const(a.f.m);
   const int tid = threadIdx.x;
   const int wave = tid / kWave;
@@ -379,13 +482,14 @@ adjoint_layout(AdjArgs a) {
   load_table(M, sTab);  // ends with a barrier
   const Ctx cx = make_ctx(M, sTab, 0);
   double* row = JAC ? nullptr : sRows + wave * np1;
+  double* lane_acc = sAcc + wave * kWave * kLdsSlots + lane;
 
   const int64_t stride = (int64_t)gridDim.x * kAdjWaves;
   for (int64_t tile = (int64_t)blockIdx.x * kAdjWaves + wave; tile < a.f.n_tiles;
        tile += stride) {
     const Ray y = load_ray<RAYS>(a.f, tile, lane / SPLIT);
     if constexpr (JAC) {
-      run_adjoint<SPLIT, CHUNK, true>(LAYOUT{}, cx, a, y, 1.0, row, lane, sub);
+      run_adjoint<SPLIT, CHUNK, true>(LAYOUT{}, cx, a, y, 1.0, row, lane, sub, lane_acc);
       continue;
     }
     double gscale = 0.0;
@@ -403,7 +507,7 @@ adjoint_layout(AdjArgs a) {
     } else if (a.mode == ZODI_ADJ_VJP_SUM) {
       gscale = y.active ? a.weights[y.r] : 0.0;
     }
-    run_adjoint<SPLIT, CHUNK, false>(LAYOUT{}, cx, a, y, gscale, row, lane, sub);
+    run_adjoint<SPLIT, CHUNK, false>(LAYOUT{}, cx, a, y, gscale, row, lane, sub, lane_acc);
   }
   if constexpr (JAC) return;
   __syncthreads();
@@ -570,6 +674,8 @@ AdjKernel pick_adj_chunk(int chunk, int split, bool jac) {
       return pick_adj_split<LAYOUT, 1>(split, jac);
     case 2:
       return pick_adj_split<LAYOUT, 2>(split, jac);
+    case 4:  // the Jacobian pass (adjoint_plan maps 4 to 2 elsewhere)
+      return jac && split == 1 ? adjoint_layout<LAYOUT, 4, 1, true> : nullptr;
     default:
       return nullptr;
   }
@@ -645,6 +751,7 @@ zodi_status validate_grad_layout(const zodi_model* m) {
 }
 
 constexpr int kDefaultChunk = 2;
+constexpr int kDefaultJacChunk = 4;
 constexpr int kDefaultBlocksPerCu = 2;
 
 struct AdjPlan {
@@ -661,7 +768,7 @@ struct AdjPlan {
 // reductions, which costs more than a partly idle last round. The Jacobian pass
 // (jac) always uses one lane per ray.
 zodi_status adjoint_plan(const zodi_launch* launch, int64_t n_rays, bool jac, AdjPlan& p) {
-  p.chunk = kDefaultChunk;
+  p.chunk = jac ? kDefaultJacChunk : kDefaultChunk;
   p.split = 0;
   p.per_cu = kDefaultBlocksPerCu;
   if (launch != nullptr) {
@@ -673,10 +780,13 @@ zodi_status adjoint_plan(const zodi_launch* launch, int64_t n_rays, bool jac, Ad
                   launch->lanes_per_ray);
     if (launch->blocks_per_cu < 0 || launch->blocks_per_cu > 64)
       return fail(ZODI_ERR_ARG, "blocks_per_cu %d outside [0, 64]", launch->blocks_per_cu);
-    if (launch->variant != 0 && launch->variant != 1 && launch->variant != 2)
-      return fail(ZODI_ERR_ARG, "adjoint variant (components per pass) must be 0, 1 or 2, got %d",
+    if (launch->variant != 0 && launch->variant != 1 && launch->variant != 2 &&
+        launch->variant != 4)
+      return fail(ZODI_ERR_ARG, "adjoint variant (components per pass) must be 0, 1, 2 or 4, got %d",
                   launch->variant);
-    if (launch->variant != 0) p.chunk = launch->variant;
+    // 4 keeps two components' accumulators in LDS, which only the Jacobian pass
+    // does; the weighted (recompute) pass runs two components per pass instead.
+    if (launch->variant != 0) p.chunk = launch->variant == 4 && !jac ? 2 : launch->variant;
     p.split = launch->lanes_per_ray;
     if (launch->blocks_per_cu > 0) p.per_cu = launch->blocks_per_cu;
   }

@@ -416,24 +416,25 @@ constexpr int geo_moments() {
 
 // Moments: sum w dR x'/R', y', z' terms (0-2); sum w dZ x', y', z' (3-5); sum w dZ (6);
 // feature: sum w dth y'/rho^2, x'/rho^2 (7, 8) and sum w dth (9).
-template <bool FEATURE>
-__device__ __forceinline__ void geo_accumulate(double* A, double w, const CompFrame& f, double dR,
+// ACC: any accumulator with add(slot, value).
+template <bool FEATURE, class ACC>
+__device__ __forceinline__ void geo_accumulate(ACC& A, double w, const CompFrame& f, double dR,
                                                double dZ, double dth) {
   const double wr = w * (dR * f.inv_r);
   const double wz = w * dZ;
-  A[0] += wr * f.x;
-  A[1] += wr * f.y;
-  A[2] += wr * f.z;
-  A[3] += wz * f.x;
-  A[4] += wz * f.y;
-  A[5] += wz * f.z;
-  A[6] += wz;
+  A.add(0, wr * f.x);
+  A.add(1, wr * f.y);
+  A.add(2, wr * f.z);
+  A.add(3, wz * f.x);
+  A.add(4, wz * f.y);
+  A.add(5, wz * f.z);
+  A.add(6, wz);
   if constexpr (FEATURE) {
     const double wt = w * dth;
     const double wtr = wt * rcp_fast(f.x * f.x + f.y * f.y);
-    A[7] += wtr * f.y;
-    A[8] += wtr * f.x;
-    A[9] += wt;
+    A.add(7, wtr * f.y);
+    A.add(8, wtr * f.x);
+    A.add(9, wt);
   }
 }
 
