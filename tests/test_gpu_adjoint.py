@@ -471,6 +471,30 @@ def test_rrm_vjp_and_jacobian_match_twin(rrm_setup):
         sensitivity_check(Jc[:, r], jt[k].numpy(), theta, 1e-8, 1e-8, f"rrm J[:, {r}] vs twin")
 
 
+def test_rrm_registry_grouping_jacobian(rrm_setup):
+    """The front end's RRM block (the three bands merged into one group, which the
+    Jacobian pass runs as one chunk with the first band summing in LDS) gives the
+    Jacobian and I of the fit's one-component-per-group block at theta0, where
+    the bands share their temperature law."""
+    from zodipy_amd import _lib
+    from zodipy_amd.runtime import _stream
+
+    _, _, theta, v, prob = rrm_setup
+    I, J = prob.jacobian(theta)
+    model = Model(prob.space._x, name="rrm-experimental")
+    merged = np.ascontiguousarray(model._packed.block)
+    assert model._packed.n_groups == 6 and prob.space.pack(theta).n_groups == 8
+    dev = torch.from_numpy(merged.view(np.uint8).copy()).to("cuda")
+    n = v.shape[1]
+    Im = torch.empty(n, dtype=torch.float64, device="cuda")
+    Jm = torch.empty((prob.n_params, n), dtype=torch.float64, device="cuda")
+    st = prob.lib.zodi_jacobian(_lib.host_ptr(merged), dev.data_ptr(), *prob._ptr_args(merged),
+                                Im.data_ptr(), Jm.data_ptr(), n, None, _stream())
+    _lib.check(st, "zodi_jacobian (registry grouping)")
+    torch.testing.assert_close(Im, I, rtol=1e-13, atol=0)
+    torch.testing.assert_close(Jm, J, rtol=1e-12, atol=1e-13 * float(J.abs().max()))
+
+
 def test_rrm_vjp_matches_central_fd(rrm_setup):
     """Central differences of the weighted sum, differenced per component before
     summing (the RRM components span orders of magnitude, and a weak one's change
