@@ -14,7 +14,8 @@
 // choice that trades recomputed node geometry for register pressure), each
 // node's geometry, B(T), dB/dT and the phase function with its C1..C3 partials
 // are formed once; every component of the chunk then adds its contributions to
-// register accumulators sized at compile time (layout templates). After a
+// accumulators sized at compile time (layout templates): registers, or in the
+// Jacobian pass's four-component chunks LDS rows for the first two. After a
 // chunk's node loop the wave sums each accumulator over its lanes with an xor
 // butterfly (every lane obtains the same bits) and lane 0 adds the sums to the
 // wave's gradient row in LDS. At the end the block
