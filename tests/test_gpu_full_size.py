@@ -8,6 +8,10 @@ depend on size, plus oracle parity on a seeded sample of the rays.
   bandpass centred at 25 um with sigma 2.5 um at 50 wavelengths (normalised
   weights), observer on a circular 1 AU orbit at phase 0.3 rad, per-component
   output.
+* Gradient: DIRBE 25 um, nside 128 (196 608 pixel centres), observer (1, 0, 0)
+  AU; target = the model with every parameter perturbed by seeded N(0, 1 %)
+  relative noise plus N(0, 0.01 MJy/sr); chi^2 with sigma = 0.01 and its
+  gradient with respect to all 78 parameters against central differences.
 * Batched gradient: six-component Planck-2018-style model at 857 GHz with a
   30-frequency top-hat bandpass of +-10 %, 1000 observers with seeded
   5000-40000 pointings each (about 22 M rays): chi^2 and its gradient in one
@@ -197,3 +201,50 @@ def test_batched_857ghz_gradient_full_size(gpu):
         print(f"  {p}: adjoint {g[i]:.10e}  fd {fd:.10e}  rel {err:.1e}", flush=True)
         worst = max(worst, err)
     assert worst <= 1e-6
+
+
+def sensitivity_check(got, want, theta, rtol, floor, what):
+    """Relative error per entry where |want| |theta| is within 1e-7 of the largest;
+    weaker entries are held to `floor` times that largest sensitivity."""
+    got, want = np.asarray(got), np.asarray(want)
+    s = np.maximum(np.abs(theta), 1e-3)
+    sens = np.abs(want) * s
+    big = sens.max()
+    err = np.abs(got - want) * s
+    strong = sens > 1e-7 * big
+    rel = np.where(strong, err / np.maximum(sens, 1e-300), 0.0)
+    assert rel.max() <= rtol, f"{what}: max relative error {rel.max():.3e} (entry {rel.argmax()})"
+    assert (err[~strong] / big).max(initial=0.0) <= floor, f"{what}: weak-entry error"
+    return float(rel.max())
+
+
+def test_dirbe25_nside128_gradient_every_parameter(gpu):
+    from zodipy_amd.fitting import LineOfSightProblem
+
+    model = Model(25 * u.micron)
+    v = healpix.pix2vec(128)
+    obs = np.array([1.0, 0.0, 0.0])
+    prob = LineOfSightProblem(model, v, obs, obs)
+    theta = prob.space.theta0
+    rng = np.random.default_rng(128)
+    theta_true = theta * (1.0 + 0.01 * rng.normal(size=theta.size))
+    data = prob.forward(theta_true) + torch.as_tensor(0.01 * rng.normal(size=v.shape[1]),
+                                                      device="cuda")
+    inv_var = 1.0 / 0.01**2
+    chi2, grad = prob.chi2_and_grad(theta, data, inv_var)
+    names = list(prob.space.names)
+
+    def chi2_fwd(th):
+        return float((((prob.forward(th) - data) ** 2) * inv_var).sum())
+
+    assert abs(float(chi2) - chi2_fwd(theta)) <= 1e-12 * float(chi2)
+    fd = np.empty_like(theta)
+    for i, p in enumerate(names):
+        # 1e-8 for the temperature law: B(T) is linear between table temperatures
+        h = (1e-8 if p in ("T_0", "delta") else 1e-6) * max(abs(theta[i]), 1e-3)
+        tp, tm = theta.copy(), theta.copy()
+        tp[i] += h
+        tm[i] -= h
+        fd[i] = (chi2_fwd(tp) - chi2_fwd(tm)) / (tp[i] - tm[i])
+    worst = sensitivity_check(grad.cpu().numpy(), fd, theta, 1e-6, 1e-6, "chi2 gradient vs FD")
+    print(f"{len(names)} parameters: worst relative error against FD {worst:.2e}")
