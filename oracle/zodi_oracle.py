@@ -11,11 +11,16 @@ follows. Inputs are ecliptic unit vectors and explicit heliocentric observer and
 Earth positions, i.e. the kernel-level inputs after the reference's ephemeris
 lookup and frame rotation (zodipy/model.py:212-251).
 
-Pinning (see tests/test_oracle_*.py): the DIRBE Zodiacal Light Prediction
-Software table (tests/dirbe_tabulated.py in the reference, 1 %), the published
-README/usage outputs (8 digits), Planck's law against mpmath, Gauss-Legendre
-exactness, brute-force sphere intersections, closed-form density values, the
-phase-function normalisation integral, and micron/GHz consistency.
+Pinning (tests/test_oracle_pinning.py): the DIRBE Zodiacal Light Prediction
+Software table (tests/dirbe_tabulated.py in the reference, 1 %); the reference's
+published usage output for an explicit observer (docs/usage.md:199-203, all 8
+digits) and its Earth-observer outputs (docs/usage.md:174, README.md:48, to the
+3e-5 that an analytic Earth ephemeris allows); Planck's law against mpmath;
+Gauss-Legendre exactness; brute-force sphere intersections; closed-form density
+values; the phase-function normalisation integral; the RRM integrand with two
+temperature laws against mpmath quadrature; the RRM calibration unit factor from
+SI constants; closed forms of the bandpass weights across units (quirk Q5) and of
+extrapolated linear tables; and micron/GHz consistency.
 """
 
 from __future__ import annotations
