@@ -58,10 +58,9 @@ __device__ __forceinline__ int64_t find_segment(const int64_t* __restrict__ seg,
   return lo;
 }
 
-// Planck table to LDS as (T_j, slope_j, B_j, 0) entries.
-// LDS entry i: (T_i, slope_i, B_i, B_{i+1} - B_i). Entries from the last
-// temperature on to ZODI_N_TEMPS hold (T_last, 0, B_last, 0), so an index
-// clamped to ZODI_N_TEMPS - 2 reads the table's end value for any n_temps.
+// Planck table to LDS: entry i is (T_i, slope_i, B_i, B_{i+1} - B_i). Entries
+// from the last temperature on to ZODI_N_TEMPS - 1 hold (T_last, 0, B_last, 0), so
+// any index from n_temps - 1 to ZODI_N_TEMPS - 1 reads the table's end value.
 __device__ __forceinline__ void load_table(ModelPtr M, double* sTab) {
   const int n = M->n_temps;
   for (int i = threadIdx.x; i < ZODI_N_TEMPS; i += blockDim.x) {
@@ -260,14 +259,14 @@ inline LayoutId match_layout(const zodi_model* m) {
 }
 
 // ---------------------------------------------------------- per-ray forward evaluation
-// Density of a component whose kind is known at compile time.
-template <int KIND>
+// Density of a component whose kind (and frame mode) is known at compile time.
+template <int KIND, int FRAME>
 __device__ __forceinline__ double density_k(ParamPtr P, uint32_t flags, const Node& nd,
                                             const FeatureDir& fd) {
   if constexpr (KIND == ZODI_INTERSTELLAR) {
     return P[6];
   } else {
-    const CompFrame f = comp_frame(P, flags, nd.X0, nd.X1, nd.X2, nd.R, nd.inv_R);
+    const CompFrame f = comp_frame<FRAME>(P, flags, nd.X0, nd.X1, nd.X2, nd.R, nd.inv_R);
     if constexpr (KIND == ZODI_CLOUD) return cloud_density(P, f);
     if constexpr (KIND == ZODI_BAND) return band_density(P, flags, f);
     if constexpr (KIND == ZODI_RING) return ring_density(P, f);
@@ -290,7 +289,8 @@ struct Ctx {
   double calibration;
 };
 
-template <int INTEGRAND, int L, int... Ks, int... J>
+// FRAMES: 2 bits per component of the group (FrameMode), component J at bits 2J.
+template <int INTEGRAND, int L, uint32_t FRAMES, int... Ks, int... J>
 __device__ __forceinline__ void group_pass(Grp<Ks...>, std::integer_sequence<int, J...>,
                                            const Ctx& cx, const FwdArgs& a, const Ray& y, int g,
                                            int first, double& total) {
@@ -319,10 +319,12 @@ __device__ __forceinline__ void group_pass(Grp<Ks...>, std::integer_sequence<int
     const ModelPtr Mk = opaque(M);
     const double xk = Mk->gl_x[k];
     const double wk = Mk->gl_w[k];
-    // Only the interstellar density does not depend on the position (and so on a
-    // NaN temperature's cause); without it the lookup skips its NaN pass-through.
-    constexpr bool kConstDensity = ((kinds[J] == ZODI_INTERSTELLAR) || ...);
-    Node nd = node_geometry<kConstDensity>(y, h, mid, xk, us, uo, mdelta, cx.tb);
+    // A NaN temperature comes from a NaN position. The Kelsall densities are NaN
+    // there too, so their groups skip the lookup's NaN pass-through; the interstellar
+    // density is a constant and the other RRM kinds mask NaN radii to 0 (as the
+    // reference's boolean masks do), so their groups keep it.
+    constexpr bool kNanFix = ((kinds[J] >= ZODI_FAN && kinds[J] <= ZODI_BROAD_BAND) || ...);
+    Node nd = node_geometry<kNanFix>(y, h, mid, xk, us, uo, mdelta, cx.tb);
     if constexpr (INTEGRAND == ZODI_INTEGRAND_KELSALL) {
       if (scatter) {
         nd.phi = phase_function(h * xk + mid, y.ux, y.uy, y.uz, nd.X0, nd.X1, nd.X2, nd.R,
@@ -332,7 +334,8 @@ __device__ __forceinline__ void group_pass(Grp<Ks...>, std::integer_sequence<int
     const double wB = wk * nd.B;
     ((acc[J] += node_term<INTEGRAND>(
           Mk->comp[first + J], Mk->comp_flags[first + J], nd, wk, wB,
-          density_k<kinds[J]>(Mk->comp[first + J], Mk->comp_flags[first + J], nd, fdir[J]),
+          density_k<kinds[J], (FRAMES >> (2 * J)) & 3>(Mk->comp[first + J],
+                                                        Mk->comp_flags[first + J], nd, fdir[J]),
           Mk->shared[ZODI_S_SOLAR])),
      ...);
   }
@@ -356,15 +359,65 @@ __device__ __forceinline__ void group_pass(Grp<Ks...>, std::integer_sequence<int
   }
 }
 
-template <int L, int INTEGRAND, class... Gs>
-__device__ __forceinline__ void run_layout(Layout<INTEGRAND, Gs...>, const Ctx& cx,
-                                           const FwdArgs& a, const Ray& y, double& total) {
-  int g = 0;
-  int first = 0;
-  ((group_pass<INTEGRAND, L>(Gs{}, std::make_integer_sequence<int, Gs::size>{}, cx, a, y, g,
-                             first, total),
-    ++g, first += Gs::size),
+// First component of group G of a layout.
+template <class... Gs>
+constexpr int group_first(int G) {
+  constexpr int sizes[] = {Gs::size...};
+  int f = 0;
+  for (int i = 0; i < G; ++i) f += sizes[i];
+  return f;
+}
+
+// FRAMES: 2 bits per component of the layout (FrameMode), component c at bits 2c.
+template <int L, uint32_t FRAMES, int INTEGRAND, class... Gs, int... G>
+__device__ __forceinline__ void run_layout_groups(Layout<INTEGRAND, Gs...>,
+                                                  std::integer_sequence<int, G...>, const Ctx& cx,
+                                                  const FwdArgs& a, const Ray& y, double& total) {
+  (group_pass<INTEGRAND, L, (FRAMES >> (2 * group_first<Gs...>(G)))>(
+       Gs{}, std::make_integer_sequence<int, Gs::size>{}, cx, a, y, G, group_first<Gs...>(G),
+       total),
    ...);
+}
+
+template <int L, uint32_t FRAMES, int INTEGRAND, class... Gs>
+__device__ __forceinline__ void run_layout(Layout<INTEGRAND, Gs...> lay, const Ctx& cx,
+                                           const FwdArgs& a, const Ray& y, double& total) {
+  run_layout_groups<L, FRAMES>(lay, std::make_integer_sequence<int, sizeof...(Gs)>{}, cx, a, y,
+                               total);
+}
+
+// Frame modes of the registry models' components: every component is centred on
+// the Sun except the Kelsall cloud (the first component of the DIRBE and Kelsall4
+// layouts). FRAMES of a layout kernel's registry variant.
+template <class LAYOUT>
+struct RegistryFrames;
+template <int INTEGRAND, class... Gs>
+struct RegistryFrames<Layout<INTEGRAND, Gs...>> {
+  static constexpr int n = group_first<Gs...>(sizeof...(Gs));
+  static constexpr uint32_t value = [] {
+    uint32_t m = 0;
+    for (int c = 0; c < n; ++c) m |= uint32_t(kFrameCentred) << (2 * c);
+    return m;
+  }();
+};
+template <>
+struct RegistryFrames<LayoutDirbe> {
+  static constexpr uint32_t value = 0x155u << 2 | kFrameOffset;  // cloud offset, 5 centred
+};
+template <>
+struct RegistryFrames<LayoutKelsall4> {
+  static constexpr uint32_t value = 0x15u << 2 | kFrameOffset;  // cloud offset, 3 centred
+};
+
+// Whether a model block may run a layout kernel with these frame modes: every
+// component fixed as centred must carry ZODI_CF_CENTERED (the offset path is
+// exact for any component).
+inline bool frames_match(const zodi_model* m, uint32_t frames) {
+  for (int c = 0; c < m->n_comps && c < 16; ++c) {
+    if (((frames >> (2 * c)) & 3u) == kFrameCentred && !(m->comp_flags[c] & ZODI_CF_CENTERED))
+      return false;
+  }
+  return true;
 }
 
 __device__ __forceinline__ Ctx make_ctx(ModelPtr M, const double* sTab, int sub) {

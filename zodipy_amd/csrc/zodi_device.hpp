@@ -130,23 +130,24 @@ __device__ __forceinline__ double sphere_distance(const RayGeom& g, double cutof
 }
 
 // Linear interpolation in the Planck table with np.interp semantics, on the
-// table's index axis (the forward kernels): u = (T - T_first)/dT, j = floor(u)
-// within [0, ZODI_N_TEMPS - 2] and B = B_j + f (B_{j+1} - B_j) with f = u - j,
-// u first clamped to [0, ZODI_N_TEMPS - 1] (np.interp's end values outside the
-// table; the LDS entries past the last temperature repeat it with a zero
-// difference). The clamps are against constants, so no operand needs
-// canonicalizing, and the temperature itself is never formed. The grid is
+// table's index axis (the forward kernels): u = (T - T_first)/dT is clamped to
+// [0, ZODI_N_TEMPS - 1] (np.interp's end values outside the table), j = floor(u)
+// and B = B_j + f (B_{j+1} - B_j) with f = u - j = fract(u). The LDS entries from
+// the last temperature on repeat it with a zero difference, so j = ZODI_N_TEMPS - 1
+// (u at the clamp) reads the end value and j needs no clamp of its own. The
+// clamps are against constants, so no operand needs canonicalizing, and the
+// temperature itself is never formed. The grid is
 // uniform (T_j = T_first + j dT up to rounding), so this is np.interp's
 // interpolation to ~1e-14 relative. NANFIX passes a NaN argument through
 // (np.interp returns NaN for a NaN temperature); a NaN T comes from a NaN
-// position, which also makes every density that depends on the position NaN, so
-// callers whose densities all do may leave the check out.
+// position, which also makes the Kelsall densities NaN, so callers whose
+// densities are all Kelsall kinds may leave the check out (the masked RRM
+// densities are 0 at a NaN position, so theirs need it).
 template <bool NANFIX = true>
 __device__ __forceinline__ double planck_lookup_u(double u, const double* __restrict__ tab) {
   const double uc = fmin(fmax(u, 0.0), (double)(ZODI_N_TEMPS - 1));
-  int j = (int)uc;
-  j = j > ZODI_N_TEMPS - 2 ? ZODI_N_TEMPS - 2 : j;
-  const double f = uc - (double)j;
+  const int j = (int)uc;
+  const double f = __builtin_amdgcn_fract(uc);  // uc - j for uc in [0, 99]
   const double* e = tab + 4 * j;
   const double B = fma(f, e[3], e[2]);
   if constexpr (NANFIX) return u == u ? B : u;
@@ -161,10 +162,18 @@ struct CompFrame {
   double x, y, z, r, inv_r, zc;
 };
 
+// Frame modes: decided per call from ZODI_CF_CENTERED, or fixed at compile time
+// by a kernel whose host side checked the flags. The offset path is exact for a
+// centred component too (X - 0 = X, and |X| is formed the same way), only slower.
+enum FrameMode { kFrameRuntime = 0, kFrameCentred = 1, kFrameOffset = 2 };
+
+template <int MODE = kFrameRuntime>
 __device__ __forceinline__ CompFrame comp_frame(ParamPtr P, uint32_t flags, double X0, double X1,
                                                 double X2, double R, double inv_R) {
   CompFrame f;
-  if (flags & ZODI_CF_CENTERED) {
+  const bool centred =
+      MODE == kFrameCentred || (MODE == kFrameRuntime && (flags & ZODI_CF_CENTERED));
+  if (centred) {
     f.x = X0;
     f.y = X1;
     f.z = X2;

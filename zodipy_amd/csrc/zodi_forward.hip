@@ -131,7 +131,9 @@ struct ForwardOccupancy<LayoutKelsall4> {
   static constexpr int value = 5;
 };
 
-template <class LAYOUT, int BLOCK, int L>
+// SPEC: every component's frame path fixed at compile time to the registry
+// models' (RegistryFrames); the host runs it only for blocks whose flags match.
+template <class LAYOUT, int BLOCK, int L, bool SPEC>
 __global__ __launch_bounds__(BLOCK)
 __attribute__((amdgpu_waves_per_eu(ForwardOccupancy<LAYOUT>::value))) void forward_layout(
     FwdArgs a) {
@@ -145,7 +147,7 @@ __attribute__((amdgpu_waves_per_eu(ForwardOccupancy<LAYOUT>::value))) void forwa
   for (int64_t tile = blockIdx.x; tile < a.n_tiles; tile += gridDim.x) {
     const Ray y = load_ray<RAYS>(a, tile, local);
     double total = 0.0;
-    run_layout<L>(LAYOUT{}, cx, a, y, total);
+    run_layout<L, SPEC ? RegistryFrames<LAYOUT>::value : 0u>(LAYOUT{}, cx, a, y, total);
     if (a.out_mode == ZODI_OUT_SUM && y.active && cx.sub == 0) a.out[y.r] = total;
   }
 }
@@ -279,36 +281,45 @@ namespace {
 
 using FwdKernel = void (*)(FwdArgs);
 
-template <class LAYOUT, int BLOCK>
+template <class LAYOUT, int BLOCK, bool SPEC>
 FwdKernel pick_layout_lanes(int L) {
   switch (L) {
     case 1:
-      return forward_layout<LAYOUT, BLOCK, 1>;
+      return forward_layout<LAYOUT, BLOCK, 1, SPEC>;
     case 2:
-      return forward_layout<LAYOUT, BLOCK, 2>;
+      return forward_layout<LAYOUT, BLOCK, 2, SPEC>;
     case 4:
-      return forward_layout<LAYOUT, BLOCK, 4>;
+      return forward_layout<LAYOUT, BLOCK, 4, SPEC>;
     case 8:
-      return forward_layout<LAYOUT, BLOCK, 8>;
+      return forward_layout<LAYOUT, BLOCK, 8, SPEC>;
     default:
       return nullptr;
   }
 }
 
-template <class LAYOUT>
-FwdKernel pick_layout(int block, int L) {
+template <class LAYOUT, bool SPEC>
+FwdKernel pick_layout_block(int block, int L) {
   switch (block) {
     case 64:
-      return pick_layout_lanes<LAYOUT, 64>(L);
+      return pick_layout_lanes<LAYOUT, 64, SPEC>(L);
     case 128:
-      return pick_layout_lanes<LAYOUT, 128>(L);
+      return pick_layout_lanes<LAYOUT, 128, SPEC>(L);
     case 256:
-      return pick_layout_lanes<LAYOUT, 256>(L);
+      return pick_layout_lanes<LAYOUT, 256, SPEC>(L);
     case 512:
-      return pick_layout_lanes<LAYOUT, 512>(L);
+      return pick_layout_lanes<LAYOUT, 512, SPEC>(L);
     default:
       return nullptr;
   }
+}
+
+// The registry-frames kernel when the block's offsets allow it, else the one that
+// tests each component's flags.
+template <class LAYOUT>
+FwdKernel pick_layout(const zodi_model* m, int block, int L) {
+  if (frames_match(m, RegistryFrames<LAYOUT>::value))
+    return pick_layout_block<LAYOUT, true>(block, L);
+  return pick_layout_block<LAYOUT, false>(block, L);
 }
 
 template <int BLOCK>
@@ -346,13 +357,13 @@ FwdKernel pick_forward(const zodi_model* m, int block, int L, int variant) {
   const LayoutId id = variant == 0 ? kLayoutGeneric : match_layout(m);
   switch (id) {
     case kLayoutDirbe:
-      return pick_layout<LayoutDirbe>(block, L);
+      return pick_layout<LayoutDirbe>(m, block, L);
     case kLayoutKelsall4:
-      return pick_layout<LayoutKelsall4>(block, L);
+      return pick_layout<LayoutKelsall4>(m, block, L);
     case kLayoutRRM:
-      return pick_layout<LayoutRRM>(block, L);
+      return pick_layout<LayoutRRM>(m, block, L);
     case kLayoutRRMSplit:
-      return pick_layout<LayoutRRMSplit>(block, L);
+      return pick_layout<LayoutRRMSplit>(m, block, L);
     default:
       return pick_generic(block, L);
   }
