@@ -54,3 +54,28 @@ def cloud_ring_model():
             emissivities={k: d.emissivities[k] for k in keep},
             albedos={k: d.albedos[k] for k in keep}))
     return name
+
+
+@pytest.fixture(scope="session")
+def planck18_style():
+    """Name of the registered six-component Planck-2018-style Kelsall model.
+
+    The registry's planck18 has four components (cloud and three bands), so the ring
+    and the feature take their emissivities from planck13; geometry, T_0 and delta
+    are DIRBE's.
+    """
+    from zodipy_amd import component_params as cp
+    from zodipy_amd import source_params as sp
+    from zodipy_amd.component import ComponentLabel as L
+    from zodipy_amd.zodiacal_light_model import Kelsall, model_registry
+
+    name = "planck18-style-6comp"
+    try:
+        model_registry.get_model(name)
+    except ValueError:
+        emiss = {**sp.EMISSIVITY_PLANCK_18, L.RING: sp.EMISSIVITY_PLANCK_13[L.RING],
+                 L.FEATURE: sp.EMISSIVITY_PLANCK_13[L.FEATURE]}
+        model_registry.register_model(name, Kelsall(
+            comps=cp.DIRBE, spectrum=sp.SPECTRUM_PLANCK, emissivities=emiss,
+            T_0=sp.T_0_DIRBE, delta=sp.DELTA_DIRBE))
+    return name

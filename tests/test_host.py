@@ -389,6 +389,35 @@ def test_pix2vec_ring_properties(nside):
     np.testing.assert_allclose(zs, z_ring, rtol=0, atol=1e-12)
 
 
+# Published RING pixel centres (Gorski et al. 2005, ApJ 622, 759, figs. 3-4), ring by
+# ring from the north: (z, pixels in the ring, longitude of the ring's first pixel in
+# units of pi/4, spacing in units of pi/4). nside 1 holds the 12 base pixels: the first
+# ring's pixels sit at longitudes pi/4 + k pi/2 and the equatorial ring starts on the
+# x axis. nside 2 alternates rings whose first pixel sits half a pixel east of
+# longitude 0 with rings that start at longitude 0.
+_HEALPIX_RINGS = {
+    1: [(2 / 3, 4, 1, 2), (0.0, 4, 0, 2), (-2 / 3, 4, 1, 2)],
+    2: [(11 / 12, 4, 1, 2), (2 / 3, 8, 0.5, 1), (1 / 3, 8, 0, 1), (0.0, 8, 0.5, 1),
+        (-1 / 3, 8, 0, 1), (-2 / 3, 8, 0.5, 1), (-11 / 12, 4, 1, 2)],
+}
+
+
+@pytest.mark.parametrize("nside", [1, 2])
+def test_pix2vec_published_pixel_centres(nside):
+    """Every pixel centre of nside 1 and 2 against the published positions, so a half-pixel
+    longitude phase error or a swapped ring fails (the zero-mean check above cannot see
+    either)."""
+    want = []
+    for z, n, first, step in _HEALPIX_RINGS[nside]:
+        phi = (first + step * np.arange(n)) * (np.pi / 4)
+        s = np.sqrt(1.0 - z * z)
+        want.append(np.stack([s * np.cos(phi), s * np.sin(phi), np.full(n, z)]))
+    want = np.concatenate(want, axis=1)
+    got = np.asarray(healpix.pix2vec(nside))
+    assert got.shape == want.shape == (3, 12 * nside * nside)
+    np.testing.assert_allclose(got, want, rtol=0, atol=1e-15)
+
+
 # --------------------------------------------------------------------- bench contract
 def test_bench_refuses_more_than_one_gpu():
     r = subprocess.run([sys.executable, str(ROOT / "bench.py"), "--gpus", "2"],
