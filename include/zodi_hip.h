@@ -335,6 +335,30 @@ zodi_status zodi_adjoint(const zodi_model* model_host, const zodi_model* model_d
                          double* workspace, int64_t workspace_bytes, const zodi_launch* launch,
                          void* stream);
 
+/*
+ * zodi_adjoint_noalloc — zodi_adjoint without a workspace: no reference
+ * counterpart (the reference has no derivatives). For callers that cannot
+ * allocate device memory (code under HIP graph capture, tight fitting loops,
+ * harnesses that forbid hipMalloc). It makes no allocation of any kind: the
+ * adjoint kernel's block partials live in a 4 MiB array that the library holds
+ * on each GPU it runs on until the process ends (at most 524288 doubles: launch
+ * blocks x (n_params + 1), e.g. 512 blocks of 257 entries with the default
+ * launch on a 256-CU GPU; larger launches return ZODI_ERR_TOO_LARGE). Every
+ * entry the kernels read is written by the same call, so nothing carries over
+ * between calls, but calls that overlap on one GPU (other streams, other host
+ * threads) share the array and race. It always takes the recompute path: in
+ * ZODI_ADJ_CHI2 mode model_out [n_rays] is required and receives I from the
+ * forward kernel. Arguments, modes, layouts and launch as in zodi_adjoint; the
+ * results equal zodi_adjoint's on its recompute path bit for bit.
+ */
+zodi_status zodi_adjoint_noalloc(const zodi_model* model_host, const zodi_model* model_dev,
+                                 const double* u, int64_t n_rays, const int64_t* seg_offsets,
+                                 int32_t n_seg, const double* obs, const double* earth,
+                                 int32_t obs_mode, const uint32_t* beyond, int32_t mode,
+                                 const double* weights, const double* target,
+                                 const double* inv_var, double inv_var_scalar, double* model_out,
+                                 double* grad, const zodi_launch* launch, void* stream);
+
 /* Targets of zodi_spectral_setup; component targets carry the component index
  * in the low byte (e.g. ZODI_SPEC_EMISSIVITY | 2). */
 enum {

@@ -31,6 +31,7 @@
 #include <algorithm>
 #include <cstdint>
 #include <cstring>
+#include <mutex>
 #include <tuple>
 #include <utility>
 
@@ -649,6 +650,12 @@ __global__ __launch_bounds__(256) void jacobian_chi2(const double* __restrict__ 
   }
 }
 
+// Block partials of the allocation-free adjoint (zodi_adjoint_noalloc): one
+// file-scope array per GPU, kept until the process ends. Its host-side name is
+// not a device address; hipGetSymbolAddress gives each device's copy.
+constexpr int64_t kScratchDoubles = int64_t(1) << 19;  // 4 MiB
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
 namespace {
 
 using AdjKernel = void (*)(AdjArgs);
@@ -895,6 +902,73 @@ AdjKernel pick_checked(const zodi_model* model_host, const AdjPlan& plan, bool j
   return st == ZODI_OK ? k : nullptr;
 }
 
+// Ray addressing shared by the entry points.
+struct RayArgs {
+  const double* u;
+  int64_t n_rays;
+  const int64_t* seg_offsets;
+  int32_t n_seg;
+  const double* obs;
+  const double* earth;
+  int32_t obs_mode;
+  const uint32_t* beyond;
+};
+
+// The recompute path: in chi^2 mode the forward kernel first writes I to vals;
+// the adjoint kernel then writes one partial row per block ([plan.grid][n_params
+// + 1], every entry written) and adjoint_reduce adds them in block order.
+zodi_status adjoint_recompute(const zodi_model* model_host, const zodi_model* model_dev,
+                              const RayArgs& r, int32_t mode, const double* weights,
+                              const double* target, const double* inv_var,
+                              double inv_var_scalar, const AdjPlan& plan, AdjKernel k,
+                              double* partials, double* vals, double* grad, void* stream) {
+  const int np = model_host->n_params;
+  const int np1 = np + 1;
+  if (mode == ZODI_ADJ_CHI2 && r.n_rays > 0) {
+    const zodi_status st =
+        zodi_forward(model_host, model_dev, r.u, r.n_rays, r.seg_offsets, r.n_seg, r.obs,
+                     r.earth, r.obs_mode, r.beyond, vals, ZODI_OUT_SUM, nullptr, stream);
+    if (st != ZODI_OK) return st;
+  }
+  AdjArgs a = make_args(model_dev, r.u, r.n_rays, r.seg_offsets, r.n_seg, r.obs, r.earth,
+                        r.obs_mode, r.beyond, plan, np);
+  a.weights = weights;
+  a.target = target;
+  a.inv_var = inv_var;
+  a.inv_var_scalar = inv_var_scalar;
+  a.model_vals = vals;
+  a.partials = partials;
+  a.mode = mode;
+  const size_t lds = (size_t)np1 * kAdjWaves * sizeof(double);
+  hipLaunchKernelGGL(k, dim3((unsigned)plan.grid), dim3(kAdjBlock), lds, (hipStream_t)stream, a);
+  const zodi_status st = check_hip(hipGetLastError(), "adjoint launch");
+  if (st != ZODI_OK) return st;
+  hipLaunchKernelGGL(adjoint_reduce, dim3(np1), dim3(256), 0, (hipStream_t)stream,
+                     (const double*)partials, (int)plan.grid, np1, grad);
+  return check_hip(hipGetLastError(), "adjoint reduce launch");
+}
+
+// Device address of this GPU's copy of `scratch`, looked up once per device with
+// that device current (the current device when first asked).
+zodi_status scratch_address(double** out) {
+  static std::mutex mu;
+  static void* addr[64] = {};
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return check_hip(e, "hipGetDevice");
+  if (dev < 0 || dev >= 64) return fail(ZODI_ERR_ARG, "device %d outside [0, 64)", dev);
+  std::lock_guard<std::mutex> lock(mu);
+  if (addr[dev] == nullptr) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return check_hip(e, "hipGetSymbolAddress(scratch)");
+    if (p == nullptr) return fail(ZODI_ERR_HIP, "hipGetSymbolAddress(scratch) gave null");
+    addr[dev] = p;
+  }
+  *out = static_cast<double*>(addr[dev]);
+  return ZODI_OK;
+}
+
 }  // namespace
 }  // namespace zodi
 
@@ -1016,27 +1090,48 @@ zodi_status zodi_adjoint(const zodi_model* model_host, const zodi_model* model_d
     return fail(ZODI_ERR_ARG, "workspace of %lld bytes needed, %lld given", (long long)need,
                 (long long)workspace_bytes);
   double* vals = model_out != nullptr ? model_out : workspace + plan.grid * np1;
-  if (mode == ZODI_ADJ_CHI2 && n_rays > 0) {
-    st = zodi_forward(model_host, model_dev, u, n_rays, seg_offsets, n_seg, obs, earth, obs_mode,
-                      beyond, vals, ZODI_OUT_SUM, nullptr, stream);
-    if (st != ZODI_OK) return st;
-  }
-  AdjArgs a = make_args(model_dev, u, n_rays, seg_offsets, n_seg, obs, earth, obs_mode, beyond,
-                        plan, np);
-  a.weights = weights;
-  a.target = target;
-  a.inv_var = inv_var;
-  a.inv_var_scalar = inv_var_scalar;
-  a.model_vals = vals;
-  a.partials = workspace;
-  a.mode = mode;
-  const size_t lds = (size_t)np1 * kAdjWaves * sizeof(double);
-  hipLaunchKernelGGL(k, dim3((unsigned)plan.grid), dim3(kAdjBlock), lds, (hipStream_t)stream, a);
-  st = check_hip(hipGetLastError(), "adjoint launch");
+  const RayArgs r = {u, n_rays, seg_offsets, n_seg, obs, earth, obs_mode, beyond};
+  return adjoint_recompute(model_host, model_dev, r, mode, weights, target, inv_var,
+                           inv_var_scalar, plan, k, workspace, vals, grad, stream);
+}
+
+zodi_status zodi_adjoint_noalloc(const zodi_model* model_host, const zodi_model* model_dev,
+                                 const double* u, int64_t n_rays, const int64_t* seg_offsets,
+                                 int32_t n_seg, const double* obs, const double* earth,
+                                 int32_t obs_mode, const uint32_t* beyond, int32_t mode,
+                                 const double* weights, const double* target,
+                                 const double* inv_var, double inv_var_scalar, double* model_out,
+                                 double* grad, const zodi_launch* launch, void* stream) {
+  zodi_status st = check_rays(model_host, model_dev, u, n_rays, seg_offsets, n_seg, obs, earth,
+                              obs_mode, beyond);
   if (st != ZODI_OK) return st;
-  hipLaunchKernelGGL(adjoint_reduce, dim3(np1), dim3(256), 0, (hipStream_t)stream,
-                     (const double*)workspace, (int)plan.grid, np1, grad);
-  return check_hip(hipGetLastError(), "adjoint reduce launch");
+  if (grad == nullptr) return fail(ZODI_ERR_ARG, "null pointer (grad)");
+  if (mode == ZODI_ADJ_CHI2) {
+    if (n_rays > 0 && (target == nullptr || model_out == nullptr))
+      return fail(ZODI_ERR_ARG, "chi2 mode needs target and model_out");
+  } else if (mode == ZODI_ADJ_VJP_SUM || mode == ZODI_ADJ_VJP_COMPS) {
+    if (n_rays > 0 && weights == nullptr) return fail(ZODI_ERR_ARG, "vjp mode needs weights");
+  } else {
+    return fail(ZODI_ERR_ARG, "unknown adjoint mode %d", mode);
+  }
+  const int np1 = model_host->n_params + 1;
+  AdjPlan plan;
+  st = adjoint_plan(launch, n_rays, false, plan);
+  if (st != ZODI_OK) return st;
+  AdjKernel k = pick_checked(model_host, plan, false, st);
+  if (k == nullptr) return st;
+  const int64_t need = plan.grid * np1;
+  if (need > kScratchDoubles)
+    return fail(ZODI_ERR_TOO_LARGE,
+                "the allocation-free adjoint holds %lld block partials, this launch needs %lld "
+                "(%lld blocks x %d entries): use fewer blocks_per_cu or zodi_adjoint",
+                (long long)kScratchDoubles, (long long)need, (long long)plan.grid, np1);
+  double* partials = nullptr;
+  st = scratch_address(&partials);
+  if (st != ZODI_OK) return st;
+  const RayArgs r = {u, n_rays, seg_offsets, n_seg, obs, earth, obs_mode, beyond};
+  return adjoint_recompute(model_host, model_dev, r, mode, weights, target, inv_var,
+                           inv_var_scalar, plan, k, partials, model_out, grad, stream);
 }
 
 }  // extern "C"
