@@ -20,7 +20,12 @@ Also reported (not the headline value):
   ecliptic unit vectors), HBM-bound, with their GB/s;
 * cpu_baseline: the NumPy oracle with the reference's loop structure and its
   fork-pool split over host cores, one pool per observer call as the reference
-  creates it, on a bounded prefix of the same observers.
+  creates it, on a bounded prefix of the same observers; cpu_baselines adds the
+  same for the single-sky workloads at full size (DIRBE 25 um at nside 16 and
+  128, the nside-256 sky with a 50-wavelength Gaussian bandpass);
+* single_sky: the nside-16 one-observer evaluation (latency, host in and out)
+  and the nside-256 full sky with the 50-wavelength bandpass and per-component
+  output (resident kernel time and samples/s).
 
 Prints one JSON line.
 """
@@ -108,7 +113,7 @@ def cpu_baseline(u, seg, obs, budget_s: float) -> dict:
     global _CPU_MODEL
     _CPU_MODEL = OracleModel(WAVELENGTH_UM * units.micron)
 
-    nproc = min(len(os.sched_getaffinity(0)), 16)
+    nproc = pool_size()
     ctx = mp.get_context("fork")
     done_rays = 0
     n_done = 0
@@ -124,23 +129,73 @@ def cpu_baseline(u, seg, obs, budget_s: float) -> dict:
         if time.perf_counter() - t0 > budget_s and n_done >= 3:
             break
     dt = time.perf_counter() - t0
-    cpu_model = "unknown"
-    try:
-        for line in Path("/proc/cpuinfo").read_text().splitlines():
-            if line.startswith("model name"):
-                cpu_model = line.split(":", 1)[1].strip()
-                break
-    except OSError:
-        pass
+    cpu_model = cpu_model_name()
     return {
         "value": done_rays * N_NODES / dt,
         "unit": "ray-node samples/s",
         "cores": nproc,
+        "affinity_cores": len(os.sched_getaffinity(0)),
+        "cores_note": POOL_NOTE,
         "kind": "port",
         "sample": (f"first {n_done} of {len(seg) - 1} observers ({done_rays} rays) of the timed "
                    f"workload, NumPy oracle, fork pool of {nproc} processes per observer call; "
                    f"{dt:.1f} s wall on {cpu_model} ({platform.machine()})"),
     }
+
+
+# The host's share of CPU on a one-GPU box is 16 cores (its process limits), whatever
+# the affinity mask reports; the pool is the reference's nprocesses at that share.
+POOL_CAP = 16
+POOL_NOTE = ("pool of min(affinity cores, 16) processes: 16 is the host CPU share of one GPU "
+             "on the measuring box")
+
+
+def pool_size() -> int:
+    return min(len(os.sched_getaffinity(0)), POOL_CAP)
+
+
+def cpu_model_name() -> str:
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def _cpu_eval_sky(args):
+    u_chunk, obs = args
+    return _CPU_MODEL.evaluate_vectors(u_chunk, obs, obs)
+
+
+def cpu_sky(model_factory, u, obs, label: str) -> dict:
+    """One reference-structured evaluate call over all of u (3, N) with one observer:
+    the oracle, np.array_split over a fork pool created for the call
+    (zodipy/model.py:182-198), per-component output. Full size, one timed call after
+    one warm-up call."""
+    import multiprocessing as mp
+
+    global _CPU_MODEL
+    _CPU_MODEL = model_factory()
+    nproc = pool_size()
+    ctx = mp.get_context("fork")
+
+    def once():
+        chunks = np.array_split(u, nproc, axis=1)
+        with ctx.Pool(nproc) as pool:
+            parts = pool.map(_cpu_eval_sky, [(c, obs) for c in chunks])
+        return np.concatenate(parts, axis=1)
+
+    once()
+    t0 = time.perf_counter()
+    once()
+    dt = time.perf_counter() - t0
+    n = u.shape[1]
+    return {"value": n * N_NODES / dt, "unit": "ray-node samples/s", "seconds": round(dt, 4),
+            "cores": nproc, "affinity_cores": len(os.sched_getaffinity(0)), "kind": "port",
+            "sample": f"{label}: all {n} rays, NumPy oracle, fork pool of {nproc} processes; "
+                      f"{cpu_model_name()}"}
 
 
 def _event_time(fn, reps: int) -> float:
@@ -157,6 +212,91 @@ def _event_time(fn, reps: int) -> float:
     e1.record(stream)
     torch.cuda.synchronize()
     return e0.elapsed_time(e1) / reps
+
+
+def single_sky(dev, reps: int = 20) -> dict:
+    """The single-sky workloads on the GPU.
+
+    nside16: DIRBE 25 um, HEALPix nside=16 (3072 pixel centres), observer = Earth =
+    (1, 0, 0) AU, per-component output: latency of one Model.evaluate_vectors call
+    from host pointings to host results (median of ``reps``) and of the resident
+    launch alone. fullsky256: DIRBE, nside=256 (786432 pixels), Gaussian bandpass
+    at 25 um with sigma 2.5 um at 50 wavelengths (normalised weights), observer on a
+    circular 1 AU orbit at phase 0.3 rad, per-component output, pointings resident:
+    event-timed launch and ray-node samples/s.
+    """
+    import torch
+
+    from zodipy_amd import Model, healpix
+    from zodipy_amd import units as zu
+    from zodipy_amd.runtime import OUT_COMPS
+
+    out = {}
+    v16 = healpix.pix2vec(16)
+    obs = np.array([1.0, 0.0, 0.0])
+    m1 = Model(25 * zu.micron, device=dev)
+    for _ in range(3):
+        m1.evaluate_vectors(v16, obspos=obs, earthpos=obs, return_comps=True)
+    lat = []
+    for _ in range(reps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        m1.evaluate_vectors(v16, obspos=obs, earthpos=obs, return_comps=True)
+        lat.append(time.perf_counter() - t0)
+    dm1 = m1.device_model
+    args1 = dm1.prepare_observers(obs, obs, np.array([0, v16.shape[1]], dtype=np.int64))
+    u16 = torch.as_tensor(v16, device=dev).contiguous()
+    o16 = torch.empty((m1.ncomps, v16.shape[1]), dtype=torch.float64, device=dev)
+    t_k = _event_time(lambda: dm1.forward_raw(u16, *args1, o16, OUT_COMPS), 50)
+    out["nside16_dirbe25_one_observer"] = {
+        "n_rays": int(v16.shape[1]), "latency_ms_median": round(1e3 * float(np.median(lat)), 4),
+        "kernel_ms": round(t_k, 4), "note": "latency: host pointings in, host results out"}
+    log(f"nside 16 (3072 rays, one observer): {1e3 * np.median(lat):.3f} ms per evaluate "
+        f"(host in/out), {t_k * 1e3:.1f} us resident launch")
+
+    x = np.linspace(17.5, 32.5, 50) * zu.micron
+    w = np.exp(-0.5 * ((x.value - 25.0) / 2.5) ** 2)
+    m2 = Model(x, weights=w, device=dev)
+    dm2 = m2.device_model
+    v256 = healpix.pix2vec(256)
+    obs2 = np.array([np.cos(0.3), np.sin(0.3), 0.0])
+    n = int(v256.shape[1])
+    u256 = torch.as_tensor(v256, device=dev).contiguous()
+    args2 = dm2.prepare_observers(obs2, obs2, np.array([0, n], dtype=np.int64))
+    o256 = torch.empty((m2.ncomps, n), dtype=torch.float64, device=dev)
+    dm2.forward_raw(u256, *args2, o256, OUT_COMPS)
+    t2 = _event_time(lambda: dm2.forward_raw(u256, *args2, o256, OUT_COMPS), reps)
+    out["fullsky_nside256_bandpass50"] = {
+        "n_rays": n, "kernel_ms": round(t2, 4),
+        "samples_per_s": n * N_NODES / (t2 / 1e3),
+        "output_GBs": round(8 * m2.ncomps * n / (t2 / 1e3) / 1e9, 1)}
+    log(f"full sky nside 256, 50-wavelength bandpass, per component: {t2:.3f} ms, "
+        f"{n * N_NODES / (t2 / 1e3):.4e} samples/s")
+    return out
+
+
+def cpu_single_sky() -> dict:
+    """CPU baselines of the single-sky workloads at full size (see single_sky), and of
+    the nside-128 forward evaluation the gradient workload differentiates."""
+    from oracle.zodi_oracle import OracleModel
+    from zodipy_amd import healpix
+    from zodipy_amd import units as zu
+
+    out = {}
+    obs = np.array([1.0, 0.0, 0.0])
+    out["nside16_dirbe25_one_observer"] = cpu_sky(
+        lambda: OracleModel(25 * zu.micron), healpix.pix2vec(16), obs, "DIRBE 25 um nside 16")
+    out["nside128_dirbe25_one_observer"] = cpu_sky(
+        lambda: OracleModel(25 * zu.micron), healpix.pix2vec(128), obs, "DIRBE 25 um nside 128")
+    x = np.linspace(17.5, 32.5, 50) * zu.micron
+    w = np.exp(-0.5 * ((x.value - 25.0) / 2.5) ** 2)
+    obs2 = np.array([np.cos(0.3), np.sin(0.3), 0.0])
+    out["fullsky_nside256_bandpass50"] = cpu_sky(
+        lambda: OracleModel(x, weights=w), healpix.pix2vec(256), obs2,
+        "DIRBE nside 256, 50-wavelength Gaussian bandpass")
+    for k, v in out.items():
+        log(f"cpu baseline {k}: {v['seconds']:.2f} s ({v['value']:.3e} samples/s)")
+    return out
 
 
 def planck18_style_model():
@@ -401,6 +541,9 @@ def main() -> int:
     ap.add_argument("--grad-only", action="store_true",
                     help="profiling aid: time only the gradient workloads and print them "
                          "(not the benchmark line)")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="profiling aid: one 1 GiB device-to-device copy (the reference "
+                         "kernel for the FETCH_SIZE/WRITE_SIZE scale factors), then exit")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -421,6 +564,14 @@ def main() -> int:
     if args.grad_only:
         torch.cuda.set_device(0)
         print(json.dumps(grad_cost(torch.device("cuda:0"), reps=3)), flush=True)
+        return 0
+    if args.calibrate_copy:
+        src = torch.empty(1 << 27, dtype=torch.float64, device="cuda:0").fill_(1.0)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log("copied 1 GiB device to device")
         return 0
 
     seg, obs = make_observers(SEED, args.observers)
@@ -470,26 +621,46 @@ def main() -> int:
 
     flops = n_rays * (N_NODES * FLOP_PER_RAY_NODE_DIRBE + FLOP_PER_RAY_SETUP_DIRBE)
     achieved = flops / (ms_per_step / 1e3) / 1e12
-    traffic = None
+    # HBM bytes and instruction counts per launch of this kernel on this workload come
+    # from rocprofv3 --pmc runs of the same command (committed summary); the counter-
+    # derived rates below divide them by this run's launch time.
+    traffic = prof = None
     tfile = ROOT / "profiles" / "forward_batched_dirbe12_traffic.json"
     if tfile.exists():
         try:
-            traffic = json.loads(tfile.read_text()).get("hbm_bytes_per_launch")
+            prof = json.loads(tfile.read_text())
+            traffic = prof.get("hbm_bytes_per_launch")
         except (OSError, json.JSONDecodeError):
-            traffic = None
+            prof = None
     roofline = {
-        "bound": "mfma",
+        "bound": "valu",
         "compute": "FP64 vector ALU (no matrix instructions); peak = MI355X FP64 78.6 TFLOP/s",
         "achieved": round(achieved, 3),
         "peak": PEAK_FP64_TFLOPS,
         "unit": "TFLOP/s",
         "frac": round(achieved / PEAK_FP64_TFLOPS, 4),
         "traffic": traffic,
+        "traffic_source": "profiles/forward_batched_dirbe12_traffic.json: rocprofv3 --pmc "
+                          "FETCH_SIZE and WRITE_SIZE of this kernel on this workload, scaled by "
+                          "a 1 GiB copy (not measured in this run)",
         "kernel": "forward_layout (DIRBE layout)",
         "flop_per_launch": flops,
         "algorithmic_bytes_per_launch": n_rays * (24 + 8),
         "output_write_GBs": round(n_rays * 8 / (ms_per_step / 1e3) / 1e9, 3),
     }
+    if prof and "counters" in prof:
+        c = prof["counters"]
+        hw = 64 * (2 * c["SQ_INSTS_VALU_FMA_F64"] + c["SQ_INSTS_VALU_MUL_F64"]
+                   + c["SQ_INSTS_VALU_ADD_F64"] + c["SQ_INSTS_VALU_TRANS_F64"])
+        issue = c["SQ_INSTS_VALU"] * 4 / (1024 * 2.4e9)  # s of VALU issue per SIMD
+        roofline["counters"] = {
+            "hw_fp64_tflops": round(hw / (ms_per_step / 1e3) / 1e12, 2),
+            "hw_fp64_frac": round(hw / (ms_per_step / 1e3) / 1e12 / PEAK_FP64_TFLOPS, 4),
+            "valu_issue_frac": round(issue / (ms_per_step / 1e3), 4),
+            "valu_per_ray_node": round(c["SQ_INSTS_VALU"] * 64 / (n_rays * N_NODES), 1),
+            "source": "profiles/forward_batched_dirbe12_traffic.json (SQ_INSTS_VALU*, "
+                      "wave-level counts per launch) over this run's launch time",
+        }
 
     batched_e2e = None
     u_host = None
@@ -570,6 +741,14 @@ def main() -> int:
         log(f"cpu baseline: {cpu['value']:.3e} samples/s ({cpu['sample']})")
 
     ingest = None if args.profile else ingest_cost(dev, n_rays)
+    sky = None if args.profile else single_sky(dev)
+    cpu_sky_base = None
+    if not args.no_cpu:
+        cpu_sky_base = cpu_single_sky()
+        if sky:
+            for k in sky:
+                if k in cpu_sky_base:
+                    sky[k]["cpu_seconds"] = cpu_sky_base[k]["seconds"]
 
     grad_ratio = None
     if not args.no_grad:
@@ -600,6 +779,8 @@ def main() -> int:
         },
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "cpu_baselines": cpu_sky_base,
+        "single_sky": sky,
         "plain": plain,
         "batched_e2e": batched_e2e,
         "batched_vs_plain": (value / plain["samples_per_s"]) if plain else None,

@@ -119,7 +119,8 @@ enum { ZODI_S_C1 = 0, ZODI_S_C2 = 1, ZODI_S_C3 = 2, ZODI_S_SOLAR = 3,
  *   FEATURE_RRM  n_0, R, sigma_r, sigma_z, theta_rad, sigma_theta_rad, A
  * Derived slots, from ZODI_P_D0 (the kernels multiply instead of dividing):
  *   CLOUD        1/(2 mu), mu/2
- *   BAND         1/delta_zeta_rad, 1/v, 1/delta_r
+ *   BAND         1/delta_zeta_rad, 1/v, 1/delta_r, then the plane normal (slots
+ *                ZODI_P_NX..NZ) times 1/delta_zeta_rad, and 1/delta_r^2
  *   RING(_RRM)   1/sigma_r^2, 1/sigma_z, 1/sigma_r
  *   FEATURE(_RRM) 1/sigma_r^2, 1/sigma_z, 1/sigma_theta_rad^2, 1/sigma_r,
  *                1/sigma_theta_rad, cos(theta_rad), sin(theta_rad)

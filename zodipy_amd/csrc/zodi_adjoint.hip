@@ -252,7 +252,8 @@ __device__ __forceinline__ void adjoint_group(Grp<Ks...>, std::integer_sequence<
     nd.X0 = s_los * y.ux + y.ox;
     nd.X1 = s_los * y.uy + y.oy;
     nd.X2 = s_los * y.uz + y.oz;
-    norm_and_inverse_node(nd.X0 * nd.X0 + nd.X1 * nd.X1 + nd.X2 * nd.X2, nd.R, nd.inv_R);
+    nd.R2 = nd.X0 * nd.X0 + nd.X1 * nd.X1 + nd.X2 * nd.X2;
+    norm_and_inverse_node(nd.R2, nd.R, nd.inv_R);
     const double logR = log_pos(nd.R);
     const double Rd = exp_fast_bounded(mdelta * logR);  // R^-delta = dT/dT0
     double dBdT;
@@ -272,7 +273,7 @@ __device__ __forceinline__ void adjoint_group(Grp<Ks...>, std::integer_sequence<
           constexpr int G = geo_moments<is_feature(KIND)>();
           const ParamPtr P = Mk->comp[first + J];
           const uint32_t flags = Mk->comp_flags[first + J];
-          const CompFrame f = comp_frame(P, flags, nd.X0, nd.X1, nd.X2, nd.R, nd.inv_R);
+          const CompFrame f = comp_frame(P, flags, nd.X0, nd.X1, nd.X2, nd.R, nd.inv_R, nd.R2);
           const DensGrad<NP> dg = density_grad<KIND>(P, flags, f, fdir[J - S], logR);
           auto& A = std::get<J - S>(acc);
           const double wn = wk * W[J - S];

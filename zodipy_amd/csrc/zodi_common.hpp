@@ -124,7 +124,7 @@ __device__ __forceinline__ void group_range(ModelPtr M, int g, const Ray& y, dou
 
 // Node geometry of one group: heliocentric position, |X|, 1/|X|, Planck value.
 struct Node {
-  double X0, X1, X2, R, inv_R, B, phi;
+  double X0, X1, X2, R, inv_R, R2, B, phi;
 };
 
 struct Tables {
@@ -146,7 +146,8 @@ __device__ __forceinline__ Node node_geometry(const Ray& y, double h, double mid
   nd.X0 = s_los * y.ux + y.ox;
   nd.X1 = s_los * y.uy + y.oy;
   nd.X2 = s_los * y.uz + y.oz;
-  norm_and_inverse_node(nd.X0 * nd.X0 + nd.X1 * nd.X1 + nd.X2 * nd.X2, nd.R, nd.inv_R);
+  nd.R2 = nd.X0 * nd.X0 + nd.X1 * nd.X1 + nd.X2 * nd.X2;
+  norm_and_inverse_node(nd.R2, nd.R, nd.inv_R);
   // T = T0 R^-delta (zodipy/blackbody.py:16-30), B = interp(T) (zodipy/brightness.py:48)
   const double u = fma(pow_radius(nd.R, mdelta), us, -uo);
   nd.B = planck_lookup_u<NANFIX>(u, tb.tab);
@@ -259,16 +260,29 @@ inline LayoutId match_layout(const zodi_model* m) {
 }
 
 // ---------------------------------------------------------- per-ray forward evaluation
-// Density of a component whose kind (and frame mode) is known at compile time.
-template <int KIND, int FRAME>
+// Static properties of a layout kernel's component, 4 bits each: the frame mode
+// (FrameMode, bits 0-1) and kFast (bit 2): a Kelsall kind evaluated with its
+// registry-only shortcuts (the band's p = 4, the cloud's floored g; see their
+// density functions).
+constexpr uint32_t kFast = 4u;
+
+template <int PROPS>
+constexpr bool fast_of() {
+  return (PROPS & kFast) != 0;
+}
+
+// Density of a component whose kind and properties are known at compile time.
+template <int KIND, int PROPS>
 __device__ __forceinline__ double density_k(ParamPtr P, uint32_t flags, const Node& nd,
                                             const FeatureDir& fd) {
   if constexpr (KIND == ZODI_INTERSTELLAR) {
     return P[6];
   } else {
-    const CompFrame f = comp_frame<FRAME>(P, flags, nd.X0, nd.X1, nd.X2, nd.R, nd.inv_R);
-    if constexpr (KIND == ZODI_CLOUD) return cloud_density(P, f);
-    if constexpr (KIND == ZODI_BAND) return band_density(P, flags, f);
+    constexpr bool FAST = fast_of<PROPS>();
+    const CompFrame f =
+        comp_frame<PROPS & 3>(P, flags, nd.X0, nd.X1, nd.X2, nd.R, nd.inv_R, nd.R2);
+    if constexpr (KIND == ZODI_CLOUD) return cloud_density<FAST>(P, f);
+    if constexpr (KIND == ZODI_BAND) return band_density<FAST>(P, flags, f);
     if constexpr (KIND == ZODI_RING) return ring_density(P, f);
     if constexpr (KIND == ZODI_FEATURE) return feature_density(P, f, fd);
     if constexpr (KIND == ZODI_FAN) return fan_density(P, f);
@@ -289,8 +303,8 @@ struct Ctx {
   double calibration;
 };
 
-// FRAMES: 2 bits per component of the group (FrameMode), component J at bits 2J.
-template <int INTEGRAND, int L, uint32_t FRAMES, int... Ks, int... J>
+// PROPS: 4 bits per component of the group (see density_k), component J at bits 4J.
+template <int INTEGRAND, int L, uint32_t PROPS, int... Ks, int... J>
 __device__ __forceinline__ void group_pass(Grp<Ks...>, std::integer_sequence<int, J...>,
                                            const Ctx& cx, const FwdArgs& a, const Ray& y, int g,
                                            int first, double& total) {
@@ -334,8 +348,8 @@ __device__ __forceinline__ void group_pass(Grp<Ks...>, std::integer_sequence<int
     const double wB = wk * nd.B;
     ((acc[J] += node_term<INTEGRAND>(
           Mk->comp[first + J], Mk->comp_flags[first + J], nd, wk, wB,
-          density_k<kinds[J], (FRAMES >> (2 * J)) & 3>(Mk->comp[first + J],
-                                                        Mk->comp_flags[first + J], nd, fdir[J]),
+          density_k<kinds[J], (PROPS >> (4 * J)) & 15>(Mk->comp[first + J],
+                                                         Mk->comp_flags[first + J], nd, fdir[J]),
           Mk->shared[ZODI_S_SOLAR])),
      ...);
   }
@@ -347,16 +361,17 @@ __device__ __forceinline__ void group_pass(Grp<Ks...>, std::integer_sequence<int
     }
   }
   const bool write = y.active && cx.sub == 0;
-#pragma unroll
-  for (int j = 0; j < C; ++j) {
-    const double e = finish<INTEGRAND>(M->comp[first + j], M->comp_flags[first + j], acc[j], h,
-                                       cx.calibration);
-    if (a.out_mode == ZODI_OUT_COMPS) {
-      if (write) a.out[(int64_t)(first + j) * a.n_rays + y.r] = e;
-    } else {
-      total = __dadd_rn(total, e);
-    }
-  }
+  (
+      [&] {
+        const double e = finish<INTEGRAND>(M->comp[first + J], M->comp_flags[first + J],
+                                           acc[J], h, cx.calibration);
+        if (a.out_mode == ZODI_OUT_COMPS) {
+          if (write) a.out[(int64_t)(first + J) * a.n_rays + y.r] = e;
+        } else {
+          total = __dadd_rn(total, e);
+        }
+      }(),
+      ...);
 }
 
 // First component of group G of a layout.
@@ -368,54 +383,62 @@ constexpr int group_first(int G) {
   return f;
 }
 
-// FRAMES: 2 bits per component of the layout (FrameMode), component c at bits 2c.
-template <int L, uint32_t FRAMES, int INTEGRAND, class... Gs, int... G>
+// PROPS: 4 bits per component of the layout (see density_k), component c at bits 4c.
+template <int L, uint32_t PROPS, int INTEGRAND, class... Gs, int... G>
 __device__ __forceinline__ void run_layout_groups(Layout<INTEGRAND, Gs...>,
                                                   std::integer_sequence<int, G...>, const Ctx& cx,
                                                   const FwdArgs& a, const Ray& y, double& total) {
-  (group_pass<INTEGRAND, L, (FRAMES >> (2 * group_first<Gs...>(G)))>(
+  (group_pass<INTEGRAND, L, (PROPS >> (4 * group_first<Gs...>(G)))>(
        Gs{}, std::make_integer_sequence<int, Gs::size>{}, cx, a, y, G, group_first<Gs...>(G),
        total),
    ...);
 }
 
-template <int L, uint32_t FRAMES, int INTEGRAND, class... Gs>
+template <int L, uint32_t PROPS, int INTEGRAND, class... Gs>
 __device__ __forceinline__ void run_layout(Layout<INTEGRAND, Gs...> lay, const Ctx& cx,
                                            const FwdArgs& a, const Ray& y, double& total) {
-  run_layout_groups<L, FRAMES>(lay, std::make_integer_sequence<int, sizeof...(Gs)>{}, cx, a, y,
-                               total);
+  run_layout_groups<L, PROPS>(lay, std::make_integer_sequence<int, sizeof...(Gs)>{}, cx, a, y,
+                              total);
 }
 
-// Frame modes of the registry models' components: every component is centred on
-// the Sun except the Kelsall cloud (the first component of the DIRBE and Kelsall4
-// layouts). FRAMES of a layout kernel's registry variant.
+// Component properties of a layout kernel's registry variant: every component is
+// centred on the Sun except the Kelsall cloud (the first component of the DIRBE
+// and Kelsall4 layouts), and the Kelsall components take their fast forms. At most
+// 8 components (4 bits each).
 template <class LAYOUT>
-struct RegistryFrames;
+struct RegistryProps;
 template <int INTEGRAND, class... Gs>
-struct RegistryFrames<Layout<INTEGRAND, Gs...>> {
+struct RegistryProps<Layout<INTEGRAND, Gs...>> {
   static constexpr int n = group_first<Gs...>(sizeof...(Gs));
+  static_assert(n <= 8, "4 bits per component in 32 bits");
   static constexpr uint32_t value = [] {
     uint32_t m = 0;
-    for (int c = 0; c < n; ++c) m |= uint32_t(kFrameCentred) << (2 * c);
+    for (int c = 0; c < n; ++c) m |= uint32_t(kFrameCentred) << (4 * c);
     return m;
   }();
 };
 template <>
-struct RegistryFrames<LayoutDirbe> {
-  static constexpr uint32_t value = 0x155u << 2 | kFrameOffset;  // cloud offset, 5 centred
+struct RegistryProps<LayoutDirbe> {  // cloud offset, 5 centred, all fast
+  static constexpr uint32_t value = 0x55555u << 4 | kFrameOffset | 0x444444u;
 };
 template <>
-struct RegistryFrames<LayoutKelsall4> {
-  static constexpr uint32_t value = 0x15u << 2 | kFrameOffset;  // cloud offset, 3 centred
+struct RegistryProps<LayoutKelsall4> {  // cloud offset, 3 centred, all fast
+  static constexpr uint32_t value = 0x555u << 4 | kFrameOffset | 0x4444u;
 };
 
-// Whether a model block may run a layout kernel with these frame modes: every
-// component fixed as centred must carry ZODI_CF_CENTERED (the offset path is
-// exact for any component).
-inline bool frames_match(const zodi_model* m, uint32_t frames) {
-  for (int c = 0; c < m->n_comps && c < 16; ++c) {
-    if (((frames >> (2 * c)) & 3u) == kFrameCentred && !(m->comp_flags[c] & ZODI_CF_CENTERED))
-      return false;
+// Whether a model block meets the preconditions of a layout kernel with these
+// properties: every component fixed as centred carries ZODI_CF_CENTERED (the
+// offset path is exact for any component); a fast band has p = 4
+// (ZODI_CF_P_IS_4) and a fast cloud gamma >= 1/2.
+inline bool props_match(const zodi_model* m, uint32_t props) {
+  for (int c = 0; c < m->n_comps && c < 8; ++c) {
+    const uint32_t pc = (props >> (4 * c)) & 15u;
+    if ((pc & 3u) == kFrameCentred && !(m->comp_flags[c] & ZODI_CF_CENTERED)) return false;
+    if (pc & kFast) {
+      const int kind = m->comp_kind[c];
+      if (kind == ZODI_BAND && !(m->comp_flags[c] & ZODI_CF_P_IS_4)) return false;
+      if (kind == ZODI_CLOUD && !(m->comp[c][9] >= 0.5)) return false;
+    }
   }
   return true;
 }

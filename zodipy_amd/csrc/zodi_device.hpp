@@ -159,7 +159,7 @@ __device__ __forceinline__ double planck_lookup_u(double u, const double* __rest
 // and 1/|X|). zc is the height above the symmetry plane:
 // Z = X'_x sin(Omega) sin(i) - X'_y cos(Omega) sin(i) + X'_z cos(i).
 struct CompFrame {
-  double x, y, z, r, inv_r, zc;
+  double x, y, z, r, inv_r, zc, r2;
 };
 
 // Frame modes: decided per call from ZODI_CF_CENTERED, or fixed at compile time
@@ -167,9 +167,10 @@ struct CompFrame {
 // centred component too (X - 0 = X, and |X| is formed the same way), only slower.
 enum FrameMode { kFrameRuntime = 0, kFrameCentred = 1, kFrameOffset = 2 };
 
+// R2 = |X|^2, the node's own sum of squares (r2 of a centred frame).
 template <int MODE = kFrameRuntime>
 __device__ __forceinline__ CompFrame comp_frame(ParamPtr P, uint32_t flags, double X0, double X1,
-                                                double X2, double R, double inv_R) {
+                                                double X2, double R, double inv_R, double R2) {
   CompFrame f;
   const bool centred =
       MODE == kFrameCentred || (MODE == kFrameRuntime && (flags & ZODI_CF_CENTERED));
@@ -179,11 +180,13 @@ __device__ __forceinline__ CompFrame comp_frame(ParamPtr P, uint32_t flags, doub
     f.z = X2;
     f.r = R;
     f.inv_r = inv_R;
+    f.r2 = R2;
   } else {
     f.x = X0 - P[ZODI_P_X0];
     f.y = X1 - P[ZODI_P_Y0];
     f.z = X2 - P[ZODI_P_Z0];
-    norm_and_inverse_node(f.x * f.x + f.y * f.y + f.z * f.z, f.r, f.inv_r);
+    f.r2 = f.x * f.x + f.y * f.y + f.z * f.z;
+    norm_and_inverse_node(f.r2, f.r, f.inv_r);
   }
   f.zc = f.x * P[ZODI_P_NX] - f.y * P[ZODI_P_NY] + f.z * P[ZODI_P_NZ];
   return f;
@@ -193,11 +196,24 @@ __device__ __forceinline__ CompFrame comp_frame(ParamPtr P, uint32_t flags, doub
 // Cloud, zodipy/number_density.py:47-73:
 // n_0 R^-alpha exp(-beta g^gamma), g = zeta^2/(2 mu) if zeta < mu else zeta - mu/2,
 // evaluated as n_0 exp(-alpha log R - beta g^gamma).
+//
+// FAST (the registry kernels, for blocks whose cloud has gamma >= 1/2, which the
+// host checks): g floored at the smallest normal double instead of selecting
+// log 0 = -inf. For
+// g below the floor the exact g^gamma < 1e-150 and the floored one < 1e-154 are
+// both absorbed in alpha log R (or, at R = 1 exactly, leave exp(-beta g^gamma)
+// = 1): the result is unchanged.
+template <bool FAST = false>
 __device__ __forceinline__ double cloud_density(ParamPtr P, const CompFrame& f) {
   const double n0 = P[6], alpha = P[7], beta = P[8], gamma = P[9], mu = P[10];
   const double inv_2mu = P[13], half_mu = P[14];
   const double zeta = fabs(f.zc) * f.inv_r;
   const double g = zeta < mu ? (zeta * zeta) * inv_2mu : zeta - half_mu;
+  if constexpr (FAST) {
+    const double gf = fmax(g, 0x1p-1022);
+    const double gg = exp_fast_bounded(gamma * log_pos(gf));
+    return n0 * exp_fast_bounded(-alpha * log_pos(f.r) - beta * gg);
+  }
   // |log R| <= 745 and g^gamma <= 1 (zeta <= 1): a bounded exponent
   return n0 * exp_fast_bounded(-alpha * log_pos(f.r) - beta * pow_pos(g, gamma));
 }
@@ -222,16 +238,35 @@ __device__ __forceinline__ double band_cutoff_exp(double x) {
 // Band, zodipy/number_density.py:76-110 (P[6] holds 3 n_0; the 1/v factor as in
 // the reference): 3 n_0/R exp(-zd^6) (1 + zd^p / v) (1 - exp(-(R/delta_r)^20)),
 // zd = zeta / delta_zeta.
+//
+// FAST (the registry kernels, for blocks whose bands have p = 4, which the host
+// checks): the plane normal pre-divided by delta_zeta (derived slots 16-18, so
+// zd = |Z'|/R), and (R/delta_r)^20 = (R^2/delta_r^2)^10 from R^2 (slot 19 =
+// 1/delta_r^2). The amplitude stays inside each density, where the reference
+// applies it: when the density underflows to a subnormal number its rounding
+// then matches the reference's.
+template <bool FAST = false>
 __device__ __forceinline__ double band_density(ParamPtr P, uint32_t flags, const CompFrame& f) {
   const double three_n0 = P[6], p = P[8];
   const double inv_dzeta = P[13], inv_v = P[14], inv_dr = P[15];
-  const double zd = (fabs(f.zc) * f.inv_r) * inv_dzeta;
+  double zd, x;
+  if constexpr (FAST) {
+    const double zs = f.x * P[ZODI_P_D0 + 3] - f.y * P[ZODI_P_D0 + 4] + f.z * P[ZODI_P_D0 + 5];
+    zd = fabs(zs) * f.inv_r;
+    const double y = f.r2 * P[ZODI_P_D0 + 6];
+    const double y2 = y * y;
+    const double y4 = y2 * y2;
+    x = (y4 * y4) * y2;
+  } else {
+    zd = (fabs(f.zc) * f.inv_r) * inv_dzeta;
+    x = pow20(f.r * inv_dr);
+  }
   const double zd2 = zd * zd;
   const double zd4 = zd2 * zd2;
   const double zd6 = zd4 * zd2;
-  const double zdp = (flags & ZODI_CF_P_IS_4) ? zd4 : pow_pos(zd, p);
+  const double zdp = (FAST || (flags & ZODI_CF_P_IS_4)) ? zd4 : pow_pos(zd, p);
   const double term3 = 1.0 + zdp * inv_v;
-  const double term4 = 1.0 - band_cutoff_exp(pow20(f.r * inv_dr));
+  const double term4 = 1.0 - band_cutoff_exp(x);
   return (three_n0 * f.inv_r) * exp_fast(-zd6) * term3 * term4;
 }
 
@@ -344,12 +379,12 @@ __host__ __device__ constexpr bool is_feature(int kind) {
   return kind == ZODI_FEATURE || kind == ZODI_FEATURE_RRM;
 }
 
-// Density of component `kind` at heliocentric X (|X| = R, 1/|X| = inv_R).
+// Density of component `kind` at heliocentric X (|X| = R, 1/|X| = inv_R, |X|^2 = R2).
 __device__ __forceinline__ double density(int kind, ParamPtr P, uint32_t flags, double X0,
                                           double X1, double X2, double R, double inv_R,
-                                          const FeatureDir& fd) {
+                                          double R2, const FeatureDir& fd) {
   if (kind == ZODI_INTERSTELLAR) return P[6];  // zodipy/number_density.py:259-264
-  const CompFrame f = comp_frame(P, flags, X0, X1, X2, R, inv_R);
+  const CompFrame f = comp_frame(P, flags, X0, X1, X2, R, inv_R, R2);
   switch (kind) {
     case ZODI_CLOUD:
       return cloud_density(P, f);

@@ -131,8 +131,9 @@ struct ForwardOccupancy<LayoutKelsall4> {
   static constexpr int value = 5;
 };
 
-// SPEC: every component's frame path fixed at compile time to the registry
-// models' (RegistryFrames); the host runs it only for blocks whose flags match.
+// SPEC: every component's frame path and form fixed at compile time to the
+// registry models' (RegistryProps); the host runs it only for blocks that meet
+// its preconditions (props_match).
 template <class LAYOUT, int BLOCK, int L, bool SPEC>
 __global__ __launch_bounds__(BLOCK)
 __attribute__((amdgpu_waves_per_eu(ForwardOccupancy<LAYOUT>::value))) void forward_layout(
@@ -147,7 +148,7 @@ __attribute__((amdgpu_waves_per_eu(ForwardOccupancy<LAYOUT>::value))) void forwa
   for (int64_t tile = blockIdx.x; tile < a.n_tiles; tile += gridDim.x) {
     const Ray y = load_ray<RAYS>(a, tile, local);
     double total = 0.0;
-    run_layout<L, SPEC ? RegistryFrames<LAYOUT>::value : 0u>(LAYOUT{}, cx, a, y, total);
+    run_layout<L, SPEC ? RegistryProps<LAYOUT>::value : 0u>(LAYOUT{}, cx, a, y, total);
     if (a.out_mode == ZODI_OUT_SUM && y.active && cx.sub == 0) a.out[y.r] = total;
   }
 }
@@ -189,7 +190,8 @@ __device__ __forceinline__ void generic_groups(const Ctx& cx, const FwdArgs& a, 
         const uint32_t flags = M->comp_flags[c];
         const int kind = M->comp_kind[c];
         const FeatureDir fd = is_feature(kind) ? feature_dir(P, y.ex, y.ey) : FeatureDir{1.0, 0.0};
-        const double dens = density(kind, P, flags, nd.X0, nd.X1, nd.X2, nd.R, nd.inv_R, fd);
+        const double dens =
+            density(kind, P, flags, nd.X0, nd.X1, nd.X2, nd.R, nd.inv_R, nd.R2, fd);
         sAcc[j][tid] += node_term<INTEGRAND>(P, flags, nd, wk, wB, dens, cx.solar);
       }
     }
@@ -273,7 +275,7 @@ __global__ __launch_bounds__(256) void number_density_kernel(const zodi_model* m
     const ParamPtr P = M->comp[c];
     const int kind = M->comp_kind[c];
     const FeatureDir fd = is_feature(kind) ? feature_dir(P, ex, ey) : FeatureDir{1.0, 0.0};
-    out[(int64_t)c * n + i] = density(kind, P, M->comp_flags[c], x0, x1, x2, R, inv_R, fd);
+    out[(int64_t)c * n + i] = density(kind, P, M->comp_flags[c], x0, x1, x2, R, inv_R, R2, fd);
   }
 }
 
@@ -313,11 +315,11 @@ FwdKernel pick_layout_block(int block, int L) {
   }
 }
 
-// The registry-frames kernel when the block's offsets allow it, else the one that
+// The registry variant when the block meets its preconditions, else the one that
 // tests each component's flags.
 template <class LAYOUT>
 FwdKernel pick_layout(const zodi_model* m, int block, int L) {
-  if (frames_match(m, RegistryFrames<LAYOUT>::value))
+  if (props_match(m, RegistryProps<LAYOUT>::value))
     return pick_layout_block<LAYOUT, true>(block, L);
   return pick_layout_block<LAYOUT, false>(block, L);
 }

@@ -42,25 +42,23 @@ __constant__ double kMathTablesConst[256 + 256 + 256] = {ZODI_EXP2_TABLE, ZODI_L
 // Taylor coefficients of log1p, r^6..r^3.
 __constant__ double kLogPoly[4] = {-1.0 / 6.0, 1.0 / 5.0, -1.0 / 4.0, 1.0 / 3.0};
 
-__shared__ double g_exp2[256];
-__shared__ double g_log_inv[256];
-__shared__ double g_log_c[256];
+// The three tables in one LDS array: 2^(j/256) at [0, 256), 1/c_j at [256, 512),
+// log(c_j) at [512, 768). Aligned to 2 KiB, so every table starts at a multiple of
+// 512 bytes and a log reads its two entries with one ds_read2st64 whose address is
+// j * 8 (immediate offsets, no add).
+__shared__ __attribute__((aligned(2048))) double g_math[768];
 
 // Copy the tables to LDS; every thread of the block must call this before use.
 __device__ __forceinline__ void stage_math_tables() {
-  for (int i = threadIdx.x; i < 256; i += blockDim.x) {
-    g_exp2[i] = kMathTablesConst[i];
-    g_log_inv[i] = kMathTablesConst[256 + i];
-    g_log_c[i] = kMathTablesConst[512 + i];
-  }
+  for (int i = threadIdx.x; i < 768; i += blockDim.x) g_math[i] = kMathTablesConst[i];
   __syncthreads();
 }
 #endif
 
 #if defined(__HIP_DEVICE_COMPILE__)
-#define ZODI_EXP2_AT(j) g_exp2[j]
-#define ZODI_LOG_INV_AT(j) g_log_inv[j]
-#define ZODI_LOG_C_AT(j) g_log_c[j]
+#define ZODI_EXP2_AT(j) g_math[j]
+#define ZODI_LOG_INV_AT(j) g_math[256 + (j)]
+#define ZODI_LOG_C_AT(j) g_math[512 + (j)]
 #define ZODI_LOG_POLY(k) kLogPoly[k]
 #else
 // Host build (accuracy tests): the same tables as plain arrays.

@@ -124,7 +124,10 @@ def _derived_slots(comp: ZodiacalComponent, kind: int) -> list[float]:
     if kind == KIND_CLOUD:
         return [1.0 / (2 * c.mu), c.mu / 2]
     if kind == KIND_BAND:
-        return [1.0 / c.delta_zeta_rad, 1.0 / c.v, 1.0 / c.delta_r]
+        # the plane normal over delta_zeta (zeta/delta_zeta = |Z'|/R) and 1/delta_r^2
+        inv = 1.0 / c.delta_zeta_rad
+        return [inv, 1.0 / c.v, 1.0 / c.delta_r, c.sin_Omega_rad * c.sin_i_rad * inv,
+                c.cos_Omega_rad * c.sin_i_rad * inv, c.cos_i_rad * inv, 1.0 / c.delta_r**2]
     if kind in (KIND_RING, KIND_RING_RRM):
         return [1.0 / c.sigma_r**2, 1.0 / c.sigma_z, 1.0 / c.sigma_r]
     if kind in (KIND_FEATURE, KIND_FEATURE_RRM):
