@@ -175,7 +175,14 @@ constexpr int layout_lds_slots(Layout<INTEGRAND, Gs...>) {
 // as a column of the Jacobian, dI_r/dtheta_p -> jac[p][r], instead of reducing
 // them over the wave; `total` receives I_r, each component's node sum finished
 // as the forward kernel finishes it and added in component order.
-template <int INTEGRAND, int SPLIT, bool JAC, int NREG, int S, int... Ks, int... J>
+//
+// PROPS: 4 bits per component of the group (component J at bits 4J; see
+// density_k): the frame mode and kFast, the registry forms of the band's
+// partials (p = 4, the pre-scaled plane normal, the R^2 cut-off). NOSCAT: the
+// block has no scattering component and no solar irradiance (the host checks),
+// so every scattering sum is an exact zero and is left out.
+template <int INTEGRAND, int SPLIT, bool JAC, int NREG, int S, uint32_t PROPS, bool NOSCAT,
+          int... Ks, int... J>
 __device__ __forceinline__ void adjoint_group(Grp<Ks...>, std::integer_sequence<int, J...>,
                                               const Ctx& cx, const AdjArgs& a, const Ray& y,
                                               int g, int first, double gscale, double* row,
@@ -221,8 +228,8 @@ __device__ __forceinline__ void adjoint_group(Grp<Ks...>, std::integer_sequence<
   // Phi enters d/d(albedo) when F != 0, and d/dF, d/dC_i when an albedo is non-zero.
   // With every albedo 0 the C_i partials are multiplied by 0, and with C2 = C3 = 0
   // Phi = N (C1 + 1) does not depend on the scattering angle: skip the angle.
-  const bool need_phase =
-      INTEGRAND == ZODI_INTEGRAND_KELSALL && (solar != 0.0 || (M->flags & 1) != 0);
+  const bool need_phase = !NOSCAT && INTEGRAND == ZODI_INTEGRAND_KELSALL &&
+                          (solar != 0.0 || (M->flags & 1) != 0);
   const bool need_angle = need_phase && ((M->flags & 1) != 0 || M->shared[ZODI_S_C2] != 0.0 ||
                                          M->shared[ZODI_S_C3] != 0.0);
   const double phi_const = M->shared[ZODI_S_PHASE_NORM] * (M->shared[ZODI_S_C1] + 1.0);
@@ -273,17 +280,20 @@ __device__ __forceinline__ void adjoint_group(Grp<Ks...>, std::integer_sequence<
           constexpr int G = geo_moments<is_feature(KIND)>();
           const ParamPtr P = Mk->comp[first + J];
           const uint32_t flags = Mk->comp_flags[first + J];
-          const CompFrame f = comp_frame(P, flags, nd.X0, nd.X1, nd.X2, nd.R, nd.inv_R, nd.R2);
-          const DensGrad<NP> dg = density_grad<KIND>(P, flags, f, fdir[J - S], logR);
+          constexpr uint32_t PC = (PROPS >> (4 * J)) & 15u;
+          const CompFrame f =
+              comp_frame<PC & 3>(P, flags, nd.X0, nd.X1, nd.X2, nd.R, nd.inv_R, nd.R2);
+          const DensGrad<NP> dg =
+              density_grad<KIND, (PC & kFast) != 0>(P, flags, f, fdir[J - S], logR);
           auto& A = std::get<J - S>(acc);
           const double wn = wk * W[J - S];
           if constexpr (INTEGRAND == ZODI_INTEGRAND_KELSALL) {
             // f = [(1 - a) eps B + a F Phi / R^2] n   (zodipy/brightness.py:21-56)
             const double alb = P[ZODI_P_ALBEDO], eps = P[ZODI_P_EMISSIVITY];
             const double oma = P[ZODI_P_ONE_MINUS_ALBEDO];
-            const double scat_unit = (solar * inv_R2) * pg.phi;  // F Phi / R^2
+            const double scat_unit = NOSCAT ? 0.0 : (solar * inv_R2) * pg.phi;  // F Phi / R^2
             double e = oma * (eps * nd.B);
-            if (flags & ZODI_CF_SCATTER) e += alb * scat_unit;
+            if (!NOSCAT && (flags & ZODI_CF_SCATTER)) e += alb * scat_unit;
             const double we = wn * e;
             geo_accumulate<is_feature(KIND)>(A, we, f, dg.dR, dg.dZ, dg.dth);
 #pragma unroll
@@ -291,7 +301,7 @@ __device__ __forceinline__ void adjoint_group(Grp<Ks...>, std::integer_sequence<
             const double wnn = wn * dg.n;
             if constexpr (JAC) {
               A.add(G + NP, (wk * nd.B) * dg.n);
-              A.add(G + NP + 1, (wk * scat_unit) * dg.n);
+              if constexpr (!NOSCAT) A.add(G + NP + 1, (wk * scat_unit) * dg.n);
             } else {
               A.add(G + NP, wnn * (oma * nd.B));                  // emissivity
               A.add(G + NP + 1, wnn * (scat_unit - eps * nd.B));  // albedo
@@ -299,7 +309,7 @@ __device__ __forceinline__ void adjoint_group(Grp<Ks...>, std::integer_sequence<
             // (1 - a) eps, packed with the same rounding as oma * eps
             sw += wnn * P[ZODI_P_SCALE];
             // F, C1..C3 enter through a Phi / R^2, which is 0 when a == 0
-            if (flags & ZODI_CF_SCATTER) {
+            if (!NOSCAT && (flags & ZODI_CF_SCATTER)) {
               const double was = wnn * alb * inv_R2;
               sh[2] += was * pg.phi;
               const double wasf = was * solar;
@@ -411,7 +421,8 @@ constexpr std::integer_sequence<int, (S + I)...> shift_seq(std::integer_sequence
 }
 
 // Group g in chunks of at most CHUNK components, from component S of the group.
-template <int INTEGRAND, int SPLIT, int CHUNK, bool JAC, int S, int... Ks>
+template <int INTEGRAND, int SPLIT, int CHUNK, bool JAC, int S, uint32_t PROPS, bool NOSCAT,
+          int... Ks>
 __device__ __forceinline__ void adjoint_chunks(Grp<Ks...> grp, const Ctx& cx, const AdjArgs& a,
                                                const Ray& y, int g, int first, double gscale,
                                                double* row, int lane, int sub, double (&sh)[6],
@@ -419,29 +430,40 @@ __device__ __forceinline__ void adjoint_chunks(Grp<Ks...> grp, const Ctx& cx, co
   constexpr int C = sizeof...(Ks);
   if constexpr (S < C) {
     constexpr int N = (C - S) < CHUNK ? (C - S) : CHUNK;
-    adjoint_group<INTEGRAND, SPLIT, JAC, reg_comps<JAC, CHUNK>(), S>(
+    adjoint_group<INTEGRAND, SPLIT, JAC, reg_comps<JAC, CHUNK>(), S, PROPS, NOSCAT>(
         grp, shift_seq<S>(std::make_integer_sequence<int, N>{}), cx, a, y, g, first, gscale, row,
         lane, sub, sh, total, lane_acc);
-    adjoint_chunks<INTEGRAND, SPLIT, CHUNK, JAC, S + N>(grp, cx, a, y, g, first, gscale, row,
-                                                        lane, sub, sh, total, lane_acc);
+    adjoint_chunks<INTEGRAND, SPLIT, CHUNK, JAC, S + N, PROPS, NOSCAT>(
+        grp, cx, a, y, g, first, gscale, row, lane, sub, sh, total, lane_acc);
   }
+}
+
+// Every group of the layout, with its first component known at compile time.
+template <int SPLIT, int CHUNK, bool JAC, uint32_t PROPS, bool NOSCAT, int INTEGRAND,
+          class... Gs, int... G>
+__device__ __forceinline__ void adjoint_groups(Layout<INTEGRAND, Gs...>,
+                                               std::integer_sequence<int, G...>, const Ctx& cx,
+                                               const AdjArgs& a, const Ray& y, double gscale,
+                                               double* row, int lane, int sub, double (&sh)[6],
+                                               double& total, double* lane_acc) {
+  (adjoint_chunks<INTEGRAND, SPLIT, CHUNK, JAC, 0, props_shift(PROPS, group_first<Gs...>(G)),
+                  NOSCAT>(Gs{}, cx, a, y, G, group_first<Gs...>(G), gscale, row, lane, sub, sh,
+                          total, lane_acc),
+   ...);
 }
 
 // Every group of the layout for one ray; then the shared-parameter sums go to the
 // wave's gradient row (wave sum, lane 0 writes) or, in the Jacobian pass, to the
 // ray's Jacobian column together with I_r.
-template <int SPLIT, int CHUNK, bool JAC, int INTEGRAND, class... Gs>
-__device__ __forceinline__ void run_adjoint(Layout<INTEGRAND, Gs...>, const Ctx& cx,
+template <int SPLIT, int CHUNK, bool JAC, uint32_t PROPS, bool NOSCAT, int INTEGRAND, class... Gs>
+__device__ __forceinline__ void run_adjoint(Layout<INTEGRAND, Gs...> lay, const Ctx& cx,
                                             const AdjArgs& a, const Ray& y, double gscale,
                                             double* row, int lane, int sub, double* lane_acc) {
   double sh[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   double total = 0.0;
-  int g = 0;
-  int first = 0;
-  ((adjoint_chunks<INTEGRAND, SPLIT, CHUNK, JAC, 0>(Gs{}, cx, a, y, g, first, gscale, row, lane,
-                                                    sub, sh, total, lane_acc),
-    ++g, first += Gs::size),
-   ...);
+  adjoint_groups<SPLIT, CHUNK, JAC, PROPS, NOSCAT>(
+      lay, std::make_integer_sequence<int, sizeof...(Gs)>{}, cx, a, y, gscale, row, lane, sub, sh,
+      total, lane_acc);
   const ModelPtr M = cx.M;
   constexpr int NSH = INTEGRAND == ZODI_INTEGRAND_KELSALL ? 6 : 1;
   if constexpr (JAC) {
@@ -462,9 +484,12 @@ __device__ __forceinline__ void run_adjoint(Layout<INTEGRAND, Gs...>, const Ctx&
 
 // One wave per tile of 64 / SPLIT rays; SPLIT lanes share a ray's nodes. JAC: the
 // Jacobian pass (one lane per ray, no weights, no reduction).
-template <class LAYOUT, int CHUNK, int SPLIT, bool JAC>
+// SPEC: the registry variant (RegistryProps; the host checks props_match).
+// NOSCAT: see adjoint_group.
+template <class LAYOUT, int CHUNK, int SPLIT, bool JAC, bool SPEC = false, bool NOSCAT = false>
 __global__ __launch_bounds__(kAdjBlock) __attribute__((amdgpu_waves_per_eu(2, 2))) void
 adjoint_layout(AdjArgs a) {
+  constexpr uint32_t PROPS = SPEC ? RegistryProps<LAYOUT>::value : 0u;
   constexpr int RAYS = kWave / SPLIT;
   extern __shared__ double sRows[];  // kAdjWaves rows of n_params + 1 (not used by JAC)
   __shared__ double sTab[4 * ZODI_N_TEMPS];
@@ -492,7 +517,8 @@ adjoint_layout(AdjArgs a) {
        tile += stride) {
     const Ray y = load_ray<RAYS>(a.f, tile, lane / SPLIT);
     if constexpr (JAC) {
-      run_adjoint<SPLIT, CHUNK, true>(LAYOUT{}, cx, a, y, 1.0, row, lane, sub, lane_acc);
+      run_adjoint<SPLIT, CHUNK, true, PROPS, NOSCAT>(LAYOUT{}, cx, a, y, 1.0, row, lane, sub,
+                                                      lane_acc);
       continue;
     }
     double gscale = 0.0;
@@ -510,7 +536,8 @@ adjoint_layout(AdjArgs a) {
     } else if (a.mode == ZODI_ADJ_VJP_SUM) {
       gscale = y.active ? a.weights[y.r] : 0.0;
     }
-    run_adjoint<SPLIT, CHUNK, false>(LAYOUT{}, cx, a, y, gscale, row, lane, sub, lane_acc);
+    run_adjoint<SPLIT, CHUNK, false, PROPS, NOSCAT>(LAYOUT{}, cx, a, y, gscale, row, lane, sub,
+                                                     lane_acc);
   }
   if constexpr (JAC) return;
   __syncthreads();
@@ -661,28 +688,42 @@ namespace {
 
 using AdjKernel = void (*)(AdjArgs);
 
-template <class LAYOUT, int CHUNK>
+template <class LAYOUT, int CHUNK, bool SPEC, bool NOSCAT>
 AdjKernel pick_adj_split(int split, bool jac) {
-  if (jac) return split == 1 ? adjoint_layout<LAYOUT, CHUNK, 1, true> : nullptr;
+  if (jac) return split == 1 ? adjoint_layout<LAYOUT, CHUNK, 1, true, SPEC, NOSCAT> : nullptr;
   switch (split) {
     case 1:
-      return adjoint_layout<LAYOUT, CHUNK, 1, false>;
+      return adjoint_layout<LAYOUT, CHUNK, 1, false, SPEC, NOSCAT>;
     case 2:
-      return adjoint_layout<LAYOUT, CHUNK, 2, false>;
+      return adjoint_layout<LAYOUT, CHUNK, 2, false, SPEC, NOSCAT>;
     case 4:
-      return adjoint_layout<LAYOUT, CHUNK, 4, false>;
+      return adjoint_layout<LAYOUT, CHUNK, 4, false, SPEC, NOSCAT>;
     default:
       return nullptr;
   }
 }
 
+// The registry variants (SPEC, and NOSCAT for Kelsall layouts) exist for the
+// default plans: four components per pass in the Jacobian pass, two in the
+// weighted pass.
 template <class LAYOUT>
-AdjKernel pick_adj_chunk(int chunk, int split, bool jac) {
+AdjKernel pick_adj_chunk(int chunk, int split, bool jac, bool spec, bool noscat) {
+  constexpr bool kKelsall = layout_integrand<LAYOUT>() == ZODI_INTEGRAND_KELSALL;
+  const bool default_plan = jac ? chunk == 4 : chunk == 2;
+  if (spec && default_plan) {
+    if (jac) {
+      if (split != 1) return nullptr;
+      if (kKelsall && noscat) return adjoint_layout<LAYOUT, 4, 1, true, true, kKelsall>;
+      return adjoint_layout<LAYOUT, 4, 1, true, true, false>;
+    }
+    if (kKelsall && noscat) return pick_adj_split<LAYOUT, 2, true, kKelsall>(split, false);
+    return pick_adj_split<LAYOUT, 2, true, false>(split, false);
+  }
   switch (chunk) {
     case 1:
-      return pick_adj_split<LAYOUT, 1>(split, jac);
+      return pick_adj_split<LAYOUT, 1, false, false>(split, jac);
     case 2:
-      return pick_adj_split<LAYOUT, 2>(split, jac);
+      return pick_adj_split<LAYOUT, 2, false, false>(split, jac);
     case 4:  // the Jacobian pass (adjoint_plan maps 4 to 2 elsewhere)
       return jac && split == 1 ? adjoint_layout<LAYOUT, 4, 1, true> : nullptr;
     default:
@@ -690,16 +731,29 @@ AdjKernel pick_adj_chunk(int chunk, int split, bool jac) {
   }
 }
 
+// A Kelsall block whose components all skip scattering and whose solar
+// irradiance is 0: every scattering sum of the adjoint is an exact zero.
+bool block_noscat(const zodi_model* m) {
+  return m->integrand == ZODI_INTEGRAND_KELSALL && (m->flags & 1) == 0 &&
+         m->shared[ZODI_S_SOLAR] == 0.0;
+}
+
+template <class LAYOUT>
+AdjKernel pick_adj_layout(const zodi_model* m, int chunk, int split, bool jac) {
+  return pick_adj_chunk<LAYOUT>(chunk, split, jac, props_match(m, RegistryProps<LAYOUT>::value),
+                                block_noscat(m));
+}
+
 AdjKernel pick_adjoint(const zodi_model* m, int chunk, int split, bool jac) {
   switch (match_layout(m)) {
     case kLayoutDirbe:
-      return pick_adj_chunk<LayoutDirbe>(chunk, split, jac);
+      return pick_adj_layout<LayoutDirbe>(m, chunk, split, jac);
     case kLayoutKelsall4:
-      return pick_adj_chunk<LayoutKelsall4>(chunk, split, jac);
+      return pick_adj_layout<LayoutKelsall4>(m, chunk, split, jac);
     case kLayoutRRM:
-      return pick_adj_chunk<LayoutRRM>(chunk, split, jac);
+      return pick_adj_layout<LayoutRRM>(m, chunk, split, jac);
     case kLayoutRRMSplit:
-      return pick_adj_chunk<LayoutRRMSplit>(chunk, split, jac);
+      return pick_adj_layout<LayoutRRMSplit>(m, chunk, split, jac);
     default:
       return nullptr;
   }

@@ -383,12 +383,28 @@ constexpr int group_first(int G) {
   return f;
 }
 
+// The property word of the components from `first` on (4 bits each).
+constexpr uint32_t props_shift(uint32_t props, int first) {
+  return first >= 8 ? 0u : props >> (4 * first);
+}
+
+template <class LAYOUT>
+struct LayoutIntegrand;
+template <int INTEGRAND, class... Gs>
+struct LayoutIntegrand<Layout<INTEGRAND, Gs...>> {
+  static constexpr int value = INTEGRAND;
+};
+template <class LAYOUT>
+constexpr int layout_integrand() {
+  return LayoutIntegrand<LAYOUT>::value;
+}
+
 // PROPS: 4 bits per component of the layout (see density_k), component c at bits 4c.
 template <int L, uint32_t PROPS, int INTEGRAND, class... Gs, int... G>
 __device__ __forceinline__ void run_layout_groups(Layout<INTEGRAND, Gs...>,
                                                   std::integer_sequence<int, G...>, const Ctx& cx,
                                                   const FwdArgs& a, const Ray& y, double& total) {
-  (group_pass<INTEGRAND, L, (PROPS >> (4 * group_first<Gs...>(G)))>(
+  (group_pass<INTEGRAND, L, props_shift(PROPS, group_first<Gs...>(G))>(
        Gs{}, std::make_integer_sequence<int, Gs::size>{}, cx, a, y, G, group_first<Gs...>(G),
        total),
    ...);

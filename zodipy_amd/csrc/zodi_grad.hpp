@@ -84,23 +84,39 @@ __device__ __forceinline__ DensGrad<5> cloud_grad(ParamPtr P, uint32_t flags, co
   return o;
 }
 
-// Band: n_0, delta_zeta [deg], v, p, delta_r.
+// Band: n_0, delta_zeta [deg], v, p, delta_r. FAST (the registry kernels, for
+// blocks whose bands have p = 4): the band density's registry form (the plane
+// normal pre-divided by delta_zeta, the cut-off from R^2), p = 4 without a test.
+template <bool FAST = false>
 __device__ __forceinline__ DensGrad<5> band_grad(ParamPtr P, uint32_t flags, const CompFrame& f) {
   const double three_n0 = P[6], dzeta = P[7], p = P[8], v = P[9], dr = P[10];
   const double inv_dzeta = P[13], inv_v = P[14], inv_dr = P[15];
-  const double zd = (fabs(f.zc) * f.inv_r) * inv_dzeta;
+  const bool p4 = FAST || (flags & ZODI_CF_P_IS_4);
+  double zd, zsign, x;
+  if constexpr (FAST) {
+    const double zs = f.x * P[ZODI_P_D0 + 3] - f.y * P[ZODI_P_D0 + 4] + f.z * P[ZODI_P_D0 + 5];
+    zd = fabs(zs) * f.inv_r;
+    zsign = sign_of(zs);  // the sign of Z
+    const double y = f.r2 * P[ZODI_P_D0 + 6];
+    const double y2 = y * y;
+    const double y4 = y2 * y2;
+    x = (y4 * y4) * y2;
+  } else {
+    zd = (fabs(f.zc) * f.inv_r) * inv_dzeta;
+    zsign = sign_of(f.zc);
+    x = pow20(f.r * inv_dr);
+  }
   const double zd2 = zd * zd;
   const double zd4 = zd2 * zd2;
   const double zd5 = zd4 * zd;
   const double zd6 = zd4 * zd2;
   // zd^p = exp(p log zd) needs log 0 = -inf; with p = 4 the log only enters the
   // p partial, which zd = 0 masks, so the special cases can be left out.
-  const double logzd = (flags & ZODI_CF_P_IS_4) ? log_pos(zd) : log_nonneg(zd);
-  const double zdp = (flags & ZODI_CF_P_IS_4) ? zd4 : (p == 0.0 ? 1.0 : exp_fast(p * logzd));
+  const double logzd = p4 ? log_pos(zd) : log_nonneg(zd);
+  const double zdp = p4 ? zd4 : (p == 0.0 ? 1.0 : exp_fast(p * logzd));
   const double t1 = three_n0 * f.inv_r;
   const double t2 = exp_fast(-zd6);
   const double t3 = 1.0 + zdp * inv_v;
-  const double x = pow20(f.r * inv_dr);
   const double ex = band_cutoff_exp(x);  // 0 where 1 - exp(-x) == 1
   const double t4 = 1.0 - ex;
   DensGrad<5> o;
@@ -108,13 +124,12 @@ __device__ __forceinline__ DensGrad<5> band_grad(ParamPtr P, uint32_t flags, con
   const double t124 = t1 * t24;
   o.n = t124 * t3;
   // d(zd^p)/dzd = p zd^(p-1) (taken as 0 at zd = 0 for p > 1)
-  const double dzdp = (flags & ZODI_CF_P_IS_4) ? 4.0 * (zd2 * zd)
-                                               : (zd > 0.0 ? p * zdp * rcp_fast(zd) : 0.0);
+  const double dzdp = p4 ? 4.0 * (zd2 * zd) : (zd > 0.0 ? p * zdp * rcp_fast(zd) : 0.0);
   const double dn_dzd = t124 * (-6.0 * zd5 * t3 + dzdp * inv_v);
   // R' d/dR' of the radial cut-off factor: 20 x exp(-x) / (1 - exp(-x)) times n
   const double rdr = (t1 * t2) * t3 * (20.0 * x) * ex;
   o.dR = (rdr - o.n - dn_dzd * zd) * f.inv_r;
-  o.dZ = dn_dzd * sign_of(f.zc) * (f.inv_r * inv_dzeta);
+  o.dZ = dn_dzd * zsign * (f.inv_r * inv_dzeta);
   o.dth = 0.0;
   o.dp[0] = 3.0 * (t24 * t3) * f.inv_r;
   o.dp[1] = -dn_dzd * zd * (inv_dzeta * kDegToRad);
@@ -331,7 +346,8 @@ __device__ __forceinline__ DensGrad<6> broad_band_grad(ParamPtr P, const CompFra
 
 // Value and parameter partials of a component whose kind is known at compile time.
 // `logR` = log |X| (heliocentric), reused where the component frame is the Sun's.
-template <int KIND>
+// FAST: the registry forms (band_grad).
+template <int KIND, bool FAST = false>
 __device__ __forceinline__ DensGrad<kind_nparams<KIND>()> density_grad(ParamPtr P, uint32_t flags,
                                                                         const CompFrame& f,
                                                                         const FeatureDir& fd,
@@ -339,7 +355,7 @@ __device__ __forceinline__ DensGrad<kind_nparams<KIND>()> density_grad(ParamPtr 
   if constexpr (KIND == ZODI_CLOUD) {
     return cloud_grad(P, flags, f, logR);
   } else if constexpr (KIND == ZODI_BAND) {
-    return band_grad(P, flags, f);
+    return band_grad<FAST>(P, flags, f);
   } else if constexpr (KIND == ZODI_RING) {
     return ring_grad(P, f);
   } else if constexpr (KIND == ZODI_FEATURE) {
