@@ -317,12 +317,14 @@ enum {
  * components evaluated per pass over a group's nodes: 0 (default: 4 in the
  * Jacobian pass, 2 otherwise), 1, 2 or 4 (four in the Jacobian pass, the third
  * and fourth with their sums in LDS; two in the weighted pass).
- * Supported layouts: the Kelsall registry models (cloud and three bands,
- * optionally followed by the ring and the feature) and the RRM model (its
- * registry grouping, or one component per group); others return
- * ZODI_ERR_MODEL. Mask edges of the RRM densities (R_inner, and R_outer where
- * it only masks) have derivative 0. The reduction order is fixed: equal inputs
- * give equal bits.
+ * Every component layout the block can describe is supported: the registry
+ * layouts (the Kelsall cloud and three bands, optionally followed by the ring
+ * and the feature; the RRM model in its registry grouping or one component per
+ * group) run kernels specialised for them, any other combination of the eleven
+ * kinds a generic kernel that takes one component per pass (variant is then
+ * ignored). Mask edges of the RRM densities (R_inner, and R_outer where it only
+ * masks) have derivative 0. The reduction order is fixed: equal inputs give
+ * equal bits.
  */
 int64_t zodi_adjoint_workspace(const zodi_model* model_host, int64_t n_rays, int32_t mode,
                                const zodi_launch* launch);

@@ -35,8 +35,8 @@ def pytest_report_header(config):
 def cloud_ring_model():
     """Name of a registered two-component Kelsall model (DIRBE's cloud and ring).
 
-    Its components form no registry layout, so the kernels run it through the
-    generic path and the adjoint rejects it.
+    Its components form no registry layout, so the forward and adjoint kernels
+    run it through their generic paths.
     """
     import dataclasses
 

@@ -371,12 +371,160 @@ def test_empty_ray_set(gpu):
     assert I.shape == (0,) and J.shape == (prob.n_params, 0)
 
 
-def test_unsupported_layout_raises(gpu, cloud_ring_model):
-    model = Model(25 * u.micron, name=cloud_ring_model)
-    v = healpix.pix2vec(2)
-    prob = LineOfSightProblem(model, v, OBS, EARTH)
-    with pytest.raises(RuntimeError, match="no adjoint for this component layout"):
-        prob.vjp(prob.space.theta0, torch.ones(v.shape[1], dtype=torch.float64, device="cuda"))
+def _custom_kelsall(labels):
+    """Name of a registered Kelsall model made of DIRBE's components ``labels`` (in
+    that order), with DIRBE's spectral tables: a layout no registry kernel covers."""
+    import dataclasses
+
+    from zodipy_amd.component import ComponentLabel
+    from zodipy_amd.zodiacal_light_model import model_registry
+
+    name = "test-kelsall-" + "-".join(labels)
+    try:
+        model_registry.get_model(name)
+    except ValueError:
+        d = model_registry.get_model("dirbe")
+        keep = [ComponentLabel(lab) for lab in labels]
+        model_registry.register_model(name, dataclasses.replace(
+            d, comps={k: d.comps[k] for k in keep},
+            emissivities={k: d.emissivities[k] for k in keep},
+            albedos={k: d.albedos[k] for k in keep}))
+    return name
+
+
+def _custom_rrm(labels):
+    """Name of a registered RRM model made of the components ``labels``: RRM ones from
+    the registry's RRM model, and "cloud" (DIRBE's Kelsall cloud, a kind no RRM
+    registry layout holds) with a temperature law of its own."""
+    from zodipy_amd import component_params as cp
+    from zodipy_amd import source_params as sp
+    from zodipy_amd.component import ComponentLabel
+    from zodipy_amd.zodiacal_light_model import RRM, model_registry
+
+    name = "test-rrm-" + "-".join(labels)
+    try:
+        model_registry.get_model(name)
+    except ValueError:
+        keep = [ComponentLabel(lab) for lab in labels]
+        comps = {k: (cp.DIRBE[k] if k == ComponentLabel.CLOUD else cp.RRM[k]) for k in keep}
+        T0 = {k: sp.T_0_RRM.get(k, 250.0) for k in keep}
+        delta = {k: sp.DELTA_RMM.get(k, 0.45) for k in keep}
+        model_registry.register_model(name, RRM(comps=comps, spectrum=sp.SPECTRUM_IRAS, T_0=T0,
+                                                delta=delta, calibration=sp.CALIBRATION_RRM))
+    return name
+
+
+def _custom_setup(name, lam, rrm=False, n=700, seed=31):
+    from zodipy_amd.zodiacal_light_model import model_registry
+
+    x = lam * u.micron
+    om = OracleModel(x, model=model_registry.get_model(name))
+    names, theta = rrm_theta_from_oracle(om) if rrm else theta_from_oracle(om)
+    v = random_dirs(n, seed)
+    prob = LineOfSightProblem(Model(x, name=name), v, OBS, EARTH)
+    assert list(prob.space.names) == names
+    assert np.array_equal(prob.space.theta0, theta)
+    return om, names, theta, v, prob
+
+
+def _check_custom_adjoint(om, names, theta, v, prob, twin_cls, what):
+    """VJP and Jacobian rows against the twin (1e-8), I against the forward kernel,
+    J^T w against the VJP, and the chi^2 paths (Jacobian, recompute) against each
+    other."""
+    tw = twin_cls(om, names)
+    w = np.random.default_rng(32).normal(size=v.shape[1])
+    g = prob.vjp(theta, torch.as_tensor(w, device="cuda")).cpu().numpy()
+    th = torch.tensor(theta, dtype=torch.float64, requires_grad=True)
+    (tw.emission(th, v, OBS, EARTH).sum(0) * torch.as_tensor(w)).sum().backward()
+    sensitivity_check(g, th.grad.numpy(), theta, 1e-8, 1e-8, f"{what}: vjp vs twin")
+    wc = np.random.default_rng(33).normal(size=(prob.n_comps, v.shape[1]))
+    gc = prob.vjp(theta, torch.as_tensor(wc, device="cuda"), return_comps=True).cpu().numpy()
+    th = torch.tensor(theta, dtype=torch.float64, requires_grad=True)
+    (tw.emission(th, v, OBS, EARTH) * torch.as_tensor(wc)).sum().backward()
+    sensitivity_check(gc, th.grad.numpy(), theta, 1e-8, 1e-8, f"{what}: vjp_comps vs twin")
+    I, J = prob.jacobian(theta)
+    torch.testing.assert_close(I, prob.forward(theta), rtol=1e-13, atol=0)
+    wt = torch.as_tensor(w, device="cuda")
+    sensitivity_check((J @ wt).cpu().numpy(), prob.vjp(theta, wt).cpu().numpy(), theta, 1e-11,
+                      1e-11, f"{what}: J^T w vs vjp")
+    rays = [0, 9, 250, 699]
+    jt = torch.autograd.functional.jacobian(
+        lambda t: tw.emission(t, v[:, rays], OBS, EARTH).sum(0),
+        torch.tensor(theta, dtype=torch.float64))
+    Jc = J.cpu().numpy()
+    for k, r in enumerate(rays):
+        sensitivity_check(Jc[:, r], jt[k].numpy(), theta, 1e-8, 1e-8, f"{what}: J[:, {r}]")
+    data = I * torch.as_tensor(1 + 0.01 * np.random.default_rng(34).normal(size=v.shape[1]),
+                               device="cuda")
+    c_j, g_j = prob.chi2_and_grad(theta, data, 3.0)
+    prob.store_jacobian = False
+    try:
+        c_r, g_r = prob.chi2_and_grad(theta, data, 3.0)
+    finally:
+        prob.store_jacobian = True
+    assert abs(float(c_j) - float(c_r)) <= 1e-11 * float(c_r)
+    sensitivity_check(g_j.cpu().numpy(), g_r.cpu().numpy(), theta, 1e-8, 1e-9,
+                      f"{what}: chi2 jacobian vs recompute")
+
+
+@pytest.mark.parametrize("labels", [("cloud", "ring"), ("cloud", "ring", "feature"),
+                                    ("band2", "feature", "cloud")],
+                         ids=["cloud-ring", "cloud-ring-feature", "band2-feature-cloud"])
+@pytest.mark.parametrize("lam", [25.0, 1.25], ids=["25um", "1.25um_scatter"])
+def test_custom_kelsall_layout_adjoint(gpu, labels, lam):
+    """Registered Kelsall models outside the registry layouts run the generic adjoint
+    (one component per pass; zodipy/zodiacal_light_model.py:116-128): two and three
+    components, and an order that splits the cloud and band 2 into separate groups,
+    with the scattering branch at 1.25 um."""
+    name = _custom_kelsall(labels)
+    args = _custom_setup(name, lam)
+    _check_custom_adjoint(*args, KelsallTwin, name)
+
+
+def test_custom_kelsall_layout_matches_central_fd(gpu):
+    """Central differences per component; T_0 and delta take a 1e-7 step (B(T) is
+    piecewise linear between the table's temperatures, see
+    test_rrm_vjp_matches_central_fd)."""
+    _, names, theta, v, prob = _custom_setup(_custom_kelsall(("cloud", "ring", "feature")),
+                                             25.0)
+    w = torch.as_tensor(np.random.default_rng(35).normal(size=v.shape[1]), device="cuda")
+    g = prob.vjp(theta, w).cpu().numpy()
+    fd = np.empty_like(theta)
+    for i in range(theta.size):
+        rel = 1e-7 if names[i] in ("T_0", "delta") else 1e-6
+        h = rel * max(abs(theta[i]), 1e-3)
+        tp, tm = theta.copy(), theta.copy()
+        tp[i] += h
+        tm[i] -= h
+        d = prob.forward(tp, return_comps=True) - prob.forward(tm, return_comps=True)
+        fd[i] = float((d * w).sum()) / (2 * h)
+    sensitivity_check(g, fd, theta, 1e-6, 1e-6, "custom layout vjp vs central FD")
+
+
+@pytest.mark.parametrize("labels", [("fan", "interstellar", "ring_rrm"), ("cloud", "fan")],
+                         ids=["fan-interstellar-ring", "cloud-fan"])
+def test_custom_rrm_layout_adjoint(gpu, labels):
+    """Registered RRM models outside the registry layouts, including one holding a
+    Kelsall cloud (an RRM integrand over a Kelsall density): the generic adjoint
+    against the RRM twin and central differences (per component, a 1e-7 step for
+    the temperature laws as in test_rrm_vjp_matches_central_fd)."""
+    from oracle.zodi_torch_twin import RRMTwin
+
+    name = _custom_rrm(labels)
+    om, names, theta, v, prob = _custom_setup(name, 25.0, rrm=True)
+    _check_custom_adjoint(om, names, theta, v, prob, RRMTwin, name)
+    w = torch.as_tensor(np.random.default_rng(36).normal(size=v.shape[1]), device="cuda")
+    g = prob.vjp(theta, w).cpu().numpy()
+    fd = np.empty_like(theta)
+    for i in range(theta.size):
+        rel = 1e-7 if names[i].startswith(("T_0.", "delta.")) else 1e-6
+        h = rel * max(abs(theta[i]), 1e-3)
+        tp, tm = theta.copy(), theta.copy()
+        tp[i] += h
+        tm[i] -= h
+        d = prob.forward(tp, return_comps=True) - prob.forward(tm, return_comps=True)
+        fd[i] = float((d * w).sum()) / (2 * h)
+    sensitivity_check(g, fd, theta, 1e-6, 1e-6, f"{name}: vjp vs central FD")
 
 
 @pytest.mark.parametrize("chunk,split",

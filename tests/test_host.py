@@ -135,9 +135,13 @@ def test_adjoint_host_validation_errors(cloud_ring_model):
     corrupt = blk.copy()
     corrupt["grad_shared"][0, 5] = -1
     assert _call_adjoint(lib, corrupt) == 2
+    # a custom layout (the generic adjoint): the same validation before any launch
     odd = np.ascontiguousarray(_packed(cloud_ring_model).block)
-    assert _call_adjoint(lib, odd) == 2
-    assert b"no adjoint" in lib.zodi_last_error()
+    assert _call_adjoint(lib, odd, launch=ct.byref(_lib.Launch(0, 3, 0, 0))) == 1
+    corrupt = odd.copy()
+    corrupt["grad_spec_b"][0, 1] = 400
+    assert _call_adjoint(lib, corrupt) == 2
+    assert b"grad_spec" in lib.zodi_last_error()
 
 
 @cpu_only
@@ -217,8 +221,9 @@ def test_jacobian_host_validation_errors(cloud_ring_model):
     assert call(blk, jac=None) == 1
     assert call(blk, launch=ct.byref(_lib.Launch(0, 2, 0, 0))) == 1
     assert b"one lane per ray" in lib.zodi_last_error()
-    odd = np.ascontiguousarray(_packed(cloud_ring_model).block)
-    assert call(odd) == 2
+    odd = np.ascontiguousarray(_packed(cloud_ring_model).block)  # the generic adjoint
+    assert call(odd, launch=ct.byref(_lib.Launch(0, 2, 0, 0))) == 1
+    assert call(odd, n=0, out=None, jac=None, ld=0) == 0
     assert call(blk, n=0, out=None, jac=None, ld=0) == 0  # nothing to do
 
 
