@@ -181,7 +181,7 @@ enum {
 typedef struct zodi_launch {
   int32_t block;          /* threads per block: 64, 128, 256, 512 */
   int32_t lanes_per_ray;  /* 1, 2, 4, 8: Gauss-Legendre nodes split over this many lanes */
-  int32_t blocks_per_cu;  /* persistent grid: resident blocks per CU; 0 = one block per tile */
+  int32_t blocks_per_cu;  /* > 0: grid-stride launch of this many blocks per CU; 0 = one block per tile */
   int32_t variant;        /* 0 generic, 1 specialised layout if available */
 } zodi_launch;
 

@@ -1,6 +1,12 @@
 """Launch-shape autotuner for the forward kernel.
 
-Candidates are (threads per block, lanes per ray, resident blocks per CU); each
+Candidates are (threads per block, lanes per ray), each launched with one block
+per tile of rays (blocks_per_cu = 0): the hardware dispatcher hands a CU a new
+block whenever one retires, which is dynamic scheduling at tile granularity
+already, so a persistent launch could only save the per-block prologue (staging
+about 9 KB of tables into LDS, ~0.1 % of a block's work) and never shortens the
+tail (one tile). Static grid-stride launches (blocks_per_cu > 0) measured 10 %
+slower, because equal tile counts per block are not equal work. Each candidate
 is timed with HIP events on a representative slice of the real batch and the
 fastest is kept. Results are cached as JSON keyed by device name, model layout
 (integrand, component kinds, groups), node count and a power-of-two size
@@ -25,10 +31,9 @@ from zodipy_amd.runtime import OUT_SUM, DeviceModel
 VARIANT_AUTO = 1
 
 CANDIDATES = [
-    (b, lanes, per_cu)
+    (b, lanes, 0)
     for b in (64, 128, 256, 512)
     for lanes in (1, 2, 4, 8)
-    for per_cu in (0, 2, 4, 8)
     if b // lanes >= 8
 ]
 

@@ -4,7 +4,7 @@ A batch is a set of rays (3, n_rays) resident in device memory, partitioned
 into contiguous segments, one per observer (``seg_offsets``). Each segment has
 its own observer and Earth position and its own cutoff test, i.e. the batch is
 equivalent to one reference ``Model.evaluate`` call per observer
-(zodipy/model.py:119-279) but runs as a single persistent-grid kernel launch.
+(zodipy/model.py:119-279) but runs as a single kernel launch.
 
 ``BatchedEvaluator`` uploads the per-observer data once, allocates the output,
 and captures the launch into a HIP graph, so ``run()`` is one graph replay on

@@ -25,12 +25,13 @@ recomputes the integrand instead of storing node values:
 * ``LineOfSightProblem.jacobian(theta)`` -- the model values and the per-ray
   Jacobian dI_r/dtheta_p themselves, for Gauss-Newton or Fisher-matrix work.
 
-Supported models: every registry model -- the Kelsall layouts (cloud and
-three bands, optionally followed by the ring and the feature) and the RRM model
-(packed one component per group, since a parameter vector may give its
-components different temperature laws). Other component combinations raise
-RuntimeError from the library. ``SpectralTableFit`` fits the spectral tables
-themselves on the device.
+Supported models: every component layout. The registry layouts -- the Kelsall
+layouts (cloud and three bands, optionally followed by the ring and the
+feature) and the RRM model (packed one component per group, since a parameter
+vector may give its components different temperature laws) -- run kernels
+specialised for them; any other combination of component kinds runs one
+generic kernel that takes a group's components one per pass over its nodes.
+``SpectralTableFit`` fits the spectral tables themselves on the device.
 """
 
 from __future__ import annotations
