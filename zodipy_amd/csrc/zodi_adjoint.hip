@@ -40,19 +40,22 @@
 namespace zodi {
 
 // The registry layouts' adjoint kernel. SPEC: the registry variant
-// (RegistryProps; the host checks props_match). NOSCAT: see adjoint_group.
-template <class LAYOUT, int CHUNK, int SPLIT, bool JAC, bool SPEC = false, bool NOSCAT = false>
+// (RegistryProps; the host checks props_match). NOSCAT: see adjoint_group. NREG:
+// components of a chunk whose accumulators stay in registers (the rest sum in
+// LDS).
+template <class LAYOUT, int CHUNK, int SPLIT, bool JAC, bool SPEC = false, bool NOSCAT = false,
+          int NREG = reg_comps<JAC, CHUNK>()>
 __global__ __launch_bounds__(kAdjBlock) __attribute__((amdgpu_waves_per_eu(2, 2))) void
 adjoint_layout(AdjArgs a) {
   constexpr uint32_t PROPS = SPEC ? RegistryProps<LAYOUT>::value : 0u;
-  // JAC chunks of more than kJacRegComps components: LDS accumulators, per wave
-  // kLdsSlots rows of 64 lanes.
-  constexpr int kLdsSlots = layout_lds_slots<CHUNK, reg_comps<JAC, CHUNK>()>(LAYOUT{});
+  // Components of a chunk beyond NREG: LDS accumulators, per wave kLdsSlots rows
+  // of 64 lanes.
+  constexpr int kLdsSlots = layout_lds_slots<CHUNK, NREG>(LAYOUT{});
   adjoint_body<SPLIT, JAC, kLdsSlots>(
       a, [&](const Ctx& cx, const Ray& y, double gscale, double* row, int lane, int sub,
              double* lane_acc) {
-        run_adjoint<SPLIT, CHUNK, JAC, PROPS, NOSCAT>(LAYOUT{}, cx, a, y, gscale, row, lane, sub,
-                                                      lane_acc);
+        run_adjoint<SPLIT, CHUNK, NREG, JAC, PROPS, NOSCAT>(LAYOUT{}, cx, a, y, gscale, row, lane,
+                                                            sub, lane_acc);
       });
 }
 
@@ -209,6 +212,12 @@ AdjKernel pick_adj_split(int split, bool jac) {
   }
 }
 
+// Components of a four-component Jacobian chunk whose accumulators stay in
+// registers in the NOSCAT variants: without the scattering sums a third one fits
+// (250 VGPRs for the DIRBE layout, 237 for Kelsall4, no spill), which halves the
+// LDS adds per node; with all four in registers the kernel spills.
+constexpr int kJacRegCompsNoscat = 3;
+
 // The registry variants (SPEC, and NOSCAT for Kelsall layouts) exist for the
 // default plans: four components per pass in the Jacobian pass, two in the
 // weighted pass.
@@ -219,7 +228,8 @@ AdjKernel pick_adj_chunk(int chunk, int split, bool jac, bool spec, bool noscat)
   if (spec && default_plan) {
     if (jac) {
       if (split != 1) return nullptr;
-      if (kKelsall && noscat) return adjoint_layout<LAYOUT, 4, 1, true, true, kKelsall>;
+      if (kKelsall && noscat)
+        return adjoint_layout<LAYOUT, 4, 1, true, true, kKelsall, kJacRegCompsNoscat>;
       return adjoint_layout<LAYOUT, 4, 1, true, true, false>;
     }
     if (kKelsall && noscat) return pick_adj_split<LAYOUT, 2, true, kKelsall>(split, false);

@@ -78,7 +78,7 @@ struct CompAcc {
 };
 
 // The same accumulators in LDS, one slot per 64-lane row (slot q of a lane at
-// p[64 q]): the components of a Jacobian chunk beyond kJacRegComps keep theirs
+// p[64 q]): the components of a Jacobian chunk beyond the register ones keep theirs
 // here (in_lds), so one pass over a group's nodes serves four components without
 // spilling registers. Each lane owns its slots, and each sum is one ds_add_f64
 // per node in program order, so the sums repeat bit for bit; they agree with the
@@ -102,7 +102,9 @@ struct AccOf<INTEGRAND, KIND, true> {
   using type = LdsAcc<INTEGRAND, KIND>;
 };
 
-// Components per Jacobian chunk whose accumulators stay in registers.
+// Components per Jacobian chunk whose accumulators stay in registers by default
+// (two components' sums with the scattering terms fill the registers that two
+// waves per SIMD leave).
 constexpr int kJacRegComps = 2;
 
 template <bool JAC, int CHUNK>
@@ -398,8 +400,8 @@ constexpr std::integer_sequence<int, (S + I)...> shift_seq(std::integer_sequence
 }
 
 // Group g in chunks of at most CHUNK components, from component S of the group.
-template <int INTEGRAND, int SPLIT, int CHUNK, bool JAC, int S, uint32_t PROPS, bool NOSCAT,
-          int... Ks>
+template <int INTEGRAND, int SPLIT, int CHUNK, int NREG, bool JAC, int S, uint32_t PROPS,
+          bool NOSCAT, int... Ks>
 __device__ __forceinline__ void adjoint_chunks(Grp<Ks...> grp, const Ctx& cx, const AdjArgs& a,
                                                const Ray& y, int g, int first, double gscale,
                                                double* row, int lane, int sub, double (&sh)[6],
@@ -407,25 +409,25 @@ __device__ __forceinline__ void adjoint_chunks(Grp<Ks...> grp, const Ctx& cx, co
   constexpr int C = sizeof...(Ks);
   if constexpr (S < C) {
     constexpr int N = (C - S) < CHUNK ? (C - S) : CHUNK;
-    adjoint_group<INTEGRAND, SPLIT, JAC, reg_comps<JAC, CHUNK>(), S, PROPS, NOSCAT>(
+    adjoint_group<INTEGRAND, SPLIT, JAC, NREG, S, PROPS, NOSCAT>(
         grp, shift_seq<S>(std::make_integer_sequence<int, N>{}), cx, a, y, g, first, gscale, row,
         lane, sub, sh, total, lane_acc);
-    adjoint_chunks<INTEGRAND, SPLIT, CHUNK, JAC, S + N, PROPS, NOSCAT>(
+    adjoint_chunks<INTEGRAND, SPLIT, CHUNK, NREG, JAC, S + N, PROPS, NOSCAT>(
         grp, cx, a, y, g, first, gscale, row, lane, sub, sh, total, lane_acc);
   }
 }
 
 // Every group of the layout, with its first component known at compile time.
-template <int SPLIT, int CHUNK, bool JAC, uint32_t PROPS, bool NOSCAT, int INTEGRAND,
+template <int SPLIT, int CHUNK, int NREG, bool JAC, uint32_t PROPS, bool NOSCAT, int INTEGRAND,
           class... Gs, int... G>
 __device__ __forceinline__ void adjoint_groups(Layout<INTEGRAND, Gs...>,
                                                std::integer_sequence<int, G...>, const Ctx& cx,
                                                const AdjArgs& a, const Ray& y, double gscale,
                                                double* row, int lane, int sub, double (&sh)[6],
                                                double& total, double* lane_acc) {
-  (adjoint_chunks<INTEGRAND, SPLIT, CHUNK, JAC, 0, props_shift(PROPS, group_first<Gs...>(G)),
-                  NOSCAT>(Gs{}, cx, a, y, G, group_first<Gs...>(G), gscale, row, lane, sub, sh,
-                          total, lane_acc),
+  (adjoint_chunks<INTEGRAND, SPLIT, CHUNK, NREG, JAC, 0,
+                  props_shift(PROPS, group_first<Gs...>(G)), NOSCAT>(
+       Gs{}, cx, a, y, G, group_first<Gs...>(G), gscale, row, lane, sub, sh, total, lane_acc),
    ...);
 }
 
@@ -455,13 +457,14 @@ __device__ __forceinline__ void adjoint_ray_tail(const Ctx& cx, const AdjArgs& a
 }
 
 // Every group of the layout for one ray, then adjoint_ray_tail.
-template <int SPLIT, int CHUNK, bool JAC, uint32_t PROPS, bool NOSCAT, int INTEGRAND, class... Gs>
+template <int SPLIT, int CHUNK, int NREG, bool JAC, uint32_t PROPS, bool NOSCAT, int INTEGRAND,
+          class... Gs>
 __device__ __forceinline__ void run_adjoint(Layout<INTEGRAND, Gs...> lay, const Ctx& cx,
                                             const AdjArgs& a, const Ray& y, double gscale,
                                             double* row, int lane, int sub, double* lane_acc) {
   double sh[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   double total = 0.0;
-  adjoint_groups<SPLIT, CHUNK, JAC, PROPS, NOSCAT>(
+  adjoint_groups<SPLIT, CHUNK, NREG, JAC, PROPS, NOSCAT>(
       lay, std::make_integer_sequence<int, sizeof...(Gs)>{}, cx, a, y, gscale, row, lane, sub, sh,
       total, lane_acc);
   adjoint_ray_tail<INTEGRAND, JAC>(cx, a, y, sh, total, row, lane);
