@@ -47,7 +47,7 @@ template <class LAYOUT, int CHUNK, int SPLIT, bool JAC, bool SPEC = false, bool 
           int NREG = reg_comps<JAC, CHUNK>()>
 __global__ __launch_bounds__(kAdjBlock) __attribute__((amdgpu_waves_per_eu(2, 2))) void
 adjoint_layout(AdjArgs a) {
-  constexpr uint32_t PROPS = SPEC ? RegistryProps<LAYOUT>::value : 0u;
+  constexpr uint32_t PROPS = SPEC ? props_without_share(RegistryProps<LAYOUT>::value) : 0u;
   // Components of a chunk beyond NREG: LDS accumulators, per wave kLdsSlots rows
   // of 64 lanes.
   constexpr int kLdsSlots = layout_lds_slots<CHUNK, NREG>(LAYOUT{});
@@ -256,7 +256,8 @@ bool block_noscat(const zodi_model* m) {
 
 template <class LAYOUT>
 AdjKernel pick_adj_layout(const zodi_model* m, int chunk, int split, bool jac) {
-  return pick_adj_chunk<LAYOUT>(chunk, split, jac, props_match(m, RegistryProps<LAYOUT>::value),
+  return pick_adj_chunk<LAYOUT>(chunk, split, jac,
+                                props_match(m, props_without_share(RegistryProps<LAYOUT>::value)),
                                 block_noscat(m));
 }
 

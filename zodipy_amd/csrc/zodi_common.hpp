@@ -261,10 +261,19 @@ inline LayoutId match_layout(const zodi_model* m) {
 
 // ---------------------------------------------------------- per-ray forward evaluation
 // Static properties of a layout kernel's component, 4 bits each: the frame mode
-// (FrameMode, bits 0-1) and kFast (bit 2): a Kelsall kind evaluated with its
+// (FrameMode, bits 0-1), kFast (bit 2): a Kelsall kind evaluated with its
 // registry-only shortcuts (the band's p = 4, the cloud's floored g; see their
-// density functions).
+// density functions), and kShareCut (bit 3, a band of the forward kernels): its
+// radial cut-off 1 - exp(-(R/delta_r)^20) is that of the first band of its group
+// (both centred and fast, the same 1/delta_r^2 bit for bit, so the same R^2 and the
+// same x), which the band reuses instead of forming it again. The DIRBE bands 1
+// and 3 share delta_r = 1.5.
 constexpr uint32_t kFast = 4u;
+constexpr uint32_t kShareCut = 8u;
+
+// The property word without the kShareCut bits (the adjoint kernels form every
+// band's cut-off and its partials themselves).
+constexpr uint32_t props_without_share(uint32_t props) { return props & 0x77777777u; }
 
 template <int PROPS>
 constexpr bool fast_of() {
@@ -272,9 +281,10 @@ constexpr bool fast_of() {
 }
 
 // Density of a component whose kind and properties are known at compile time.
-template <int KIND, int PROPS>
+// CUT (bands): see band_density; `cut` carries the shared cut-off within a node.
+template <int KIND, int PROPS, int CUT = kCutOwn>
 __device__ __forceinline__ double density_k(ParamPtr P, uint32_t flags, const Node& nd,
-                                            const FeatureDir& fd) {
+                                            const FeatureDir& fd, double& cut) {
   if constexpr (KIND == ZODI_INTERSTELLAR) {
     return P[6];
   } else {
@@ -282,7 +292,7 @@ __device__ __forceinline__ double density_k(ParamPtr P, uint32_t flags, const No
     const CompFrame f =
         comp_frame<PROPS & 3>(P, flags, nd.X0, nd.X1, nd.X2, nd.R, nd.inv_R, nd.R2);
     if constexpr (KIND == ZODI_CLOUD) return cloud_density<FAST>(P, f);
-    if constexpr (KIND == ZODI_BAND) return band_density<FAST>(P, flags, f);
+    if constexpr (KIND == ZODI_BAND) return band_density<FAST, CUT>(P, flags, f, cut);
     if constexpr (KIND == ZODI_RING) return ring_density(P, f);
     if constexpr (KIND == ZODI_FEATURE) return feature_density(P, f, fd);
     if constexpr (KIND == ZODI_FAN) return fan_density(P, f);
@@ -302,6 +312,25 @@ struct Ctx {
   double solar;
   double calibration;
 };
+
+// Index of the first band among a group's kinds (-1 without one).
+template <int... Ks>
+constexpr int first_band() {
+  constexpr int kinds[] = {Ks...};
+  for (int j = 0; j < (int)sizeof...(Ks); ++j)
+    if (kinds[j] == ZODI_BAND) return j;
+  return -1;
+}
+
+// The cut-off mode (BandCut) of component J of a group with these properties.
+template <uint32_t PROPS, int J, int... Ks>
+constexpr int cut_mode() {
+  constexpr int C = sizeof...(Ks);
+  bool shared = false;
+  for (int j = 0; j < C && j < 8; ++j) shared = shared || ((PROPS >> (4 * j)) & kShareCut);
+  if ((PROPS >> (4 * J)) & kShareCut) return kCutReuse;
+  return shared && J == first_band<Ks...>() ? kCutPublish : kCutOwn;
+}
 
 // PROPS: 4 bits per component of the group (see density_k), component J at bits 4J.
 template <int INTEGRAND, int L, uint32_t PROPS, int... Ks, int... J>
@@ -346,10 +375,11 @@ __device__ __forceinline__ void group_pass(Grp<Ks...>, std::integer_sequence<int
       }
     }
     const double wB = wk * nd.B;
+    double cut = 0.0;  // a band's cut-off, for the bands that share it (in order)
     ((acc[J] += node_term<INTEGRAND>(
           Mk->comp[first + J], Mk->comp_flags[first + J], nd, wk, wB,
-          density_k<kinds[J], (PROPS >> (4 * J)) & 15>(Mk->comp[first + J],
-                                                         Mk->comp_flags[first + J], nd, fdir[J]),
+          density_k<kinds[J], (PROPS >> (4 * J)) & 15, cut_mode<PROPS, J, Ks...>()>(
+              Mk->comp[first + J], Mk->comp_flags[first + J], nd, fdir[J], cut),
           Mk->shared[ZODI_S_SOLAR])),
      ...);
   }
@@ -434,18 +464,20 @@ struct RegistryProps<Layout<INTEGRAND, Gs...>> {
   }();
 };
 template <>
-struct RegistryProps<LayoutDirbe> {  // cloud offset, 5 centred, all fast
-  static constexpr uint32_t value = 0x55555u << 4 | kFrameOffset | 0x444444u;
+struct RegistryProps<LayoutDirbe> {  // cloud offset, 5 centred, all fast; band 3 shares band 1's cut-off
+  static constexpr uint32_t value = (0x55555u << 4 | kFrameOffset | 0x444444u) | kShareCut << 12;
 };
 template <>
-struct RegistryProps<LayoutKelsall4> {  // cloud offset, 3 centred, all fast
-  static constexpr uint32_t value = 0x555u << 4 | kFrameOffset | 0x4444u;
+struct RegistryProps<LayoutKelsall4> {  // cloud offset, 3 centred, all fast; band 3 shares band 1's cut-off
+  static constexpr uint32_t value = (0x555u << 4 | kFrameOffset | 0x4444u) | kShareCut << 12;
 };
 
 // Whether a model block meets the preconditions of a layout kernel with these
 // properties: every component fixed as centred carries ZODI_CF_CENTERED (the
 // offset path is exact for any component); a fast band has p = 4
-// (ZODI_CF_P_IS_4) and a fast cloud gamma >= 1/2.
+// (ZODI_CF_P_IS_4) and a fast cloud gamma >= 1/2; a band with kShareCut and the
+// first band of its group are both centred and fast and hold the same
+// 1/delta_r^2.
 inline bool props_match(const zodi_model* m, uint32_t props) {
   for (int c = 0; c < m->n_comps && c < 8; ++c) {
     const uint32_t pc = (props >> (4 * c)) & 15u;
@@ -454,6 +486,19 @@ inline bool props_match(const zodi_model* m, uint32_t props) {
       const int kind = m->comp_kind[c];
       if (kind == ZODI_BAND && !(m->comp_flags[c] & ZODI_CF_P_IS_4)) return false;
       if (kind == ZODI_CLOUD && !(m->comp[c][9] >= 0.5)) return false;
+    }
+    if (pc & kShareCut) {
+      int g = 0;
+      while (g < m->n_groups && c >= m->group_first[g] + m->group_count[g]) ++g;
+      if (g == m->n_groups) return false;
+      int b = m->group_first[g];
+      while (b < c && m->comp_kind[b] != ZODI_BAND) ++b;
+      if (b >= c || m->comp_kind[c] != ZODI_BAND) return false;
+      const uint32_t pb = (props >> (4 * b)) & 15u;
+      const uint32_t need = kFast | kFrameCentred;
+      if ((pc & 7u) != need || (pb & 7u) != need) return false;
+      if (!(m->comp_flags[b] & ZODI_CF_CENTERED)) return false;
+      if (m->comp[c][ZODI_P_D0 + 6] != m->comp[b][ZODI_P_D0 + 6]) return false;
     }
   }
   return true;

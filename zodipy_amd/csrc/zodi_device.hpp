@@ -245,29 +245,50 @@ __device__ __forceinline__ double band_cutoff_exp(double x) {
 // 1/delta_r^2). The amplitude stays inside each density, where the reference
 // applies it: when the density underflows to a subnormal number its rounding
 // then matches the reference's.
-template <bool FAST = false>
-__device__ __forceinline__ double band_density(ParamPtr P, uint32_t flags, const CompFrame& f) {
+//
+// CUT: kCutOwn forms the cut-off term 1 - exp(-x); kCutPublish also stores it in
+// `cut`; kCutReuse takes it from `cut` (a band whose cut-off the host has checked to
+// be bit for bit that of a band published earlier in the same node).
+enum BandCut { kCutOwn = 0, kCutPublish = 1, kCutReuse = 2 };
+
+template <bool FAST = false, int CUT = kCutOwn>
+__device__ __forceinline__ double band_density(ParamPtr P, uint32_t flags, const CompFrame& f,
+                                               double& cut) {
   const double three_n0 = P[6], p = P[8];
   const double inv_dzeta = P[13], inv_v = P[14], inv_dr = P[15];
-  double zd, x;
+  double zd, x = 0.0;
   if constexpr (FAST) {
     const double zs = f.x * P[ZODI_P_D0 + 3] - f.y * P[ZODI_P_D0 + 4] + f.z * P[ZODI_P_D0 + 5];
     zd = fabs(zs) * f.inv_r;
-    const double y = f.r2 * P[ZODI_P_D0 + 6];
-    const double y2 = y * y;
-    const double y4 = y2 * y2;
-    x = (y4 * y4) * y2;
+    if constexpr (CUT != kCutReuse) {
+      const double y = f.r2 * P[ZODI_P_D0 + 6];
+      const double y2 = y * y;
+      const double y4 = y2 * y2;
+      x = (y4 * y4) * y2;
+    }
   } else {
     zd = (fabs(f.zc) * f.inv_r) * inv_dzeta;
-    x = pow20(f.r * inv_dr);
+    if constexpr (CUT != kCutReuse) x = pow20(f.r * inv_dr);
   }
   const double zd2 = zd * zd;
   const double zd4 = zd2 * zd2;
   const double zd6 = zd4 * zd2;
   const double zdp = (FAST || (flags & ZODI_CF_P_IS_4)) ? zd4 : pow_pos(zd, p);
   const double term3 = 1.0 + zdp * inv_v;
-  const double term4 = 1.0 - band_cutoff_exp(x);
+  double term4;
+  if constexpr (CUT == kCutReuse) {
+    term4 = cut;
+  } else {
+    term4 = 1.0 - band_cutoff_exp(x);
+    if constexpr (CUT == kCutPublish) cut = term4;
+  }
   return (three_n0 * f.inv_r) * exp_fast(-zd6) * term3 * term4;
+}
+
+template <bool FAST = false>
+__device__ __forceinline__ double band_density(ParamPtr P, uint32_t flags, const CompFrame& f) {
+  double cut = 0.0;
+  return band_density<FAST, kCutOwn>(P, flags, f, cut);
 }
 
 // Ring, zodipy/number_density.py:113-139: n_0 exp(-(R - R_r)^2/sigma_r^2 - |Z|/sigma_z).
