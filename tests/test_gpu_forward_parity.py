@@ -521,3 +521,36 @@ def test_grid_number_density_public_api(gpu):
     assert not np.array_equal(moved[5], got[5])
     assert_parity(moved, OracleModel(25 * u.micron).number_density(pts, earth2).reshape(
         6, 5, 7, 3))
+
+
+@pytest.mark.parametrize("lanes", [1, 2])
+def test_registry_variant_with_unshared_band_cutoff(gpu, lanes):
+    """The DIRBE bands 1 and 3 share delta_r = 1.5, so the registry kernel with one
+    lane per ray forms their radial cut-off once. A parameter update that moves band
+    3's delta_r (as a fit does) must leave that sharing out: parity with the oracle
+    of the updated model, and the output changes only in band 3."""
+    import dataclasses
+
+    from zodipy_amd.component import ComponentLabel
+    from zodipy_amd.zodiacal_light_model import model_registry
+
+    v = random_dirs(2000, seed=41)
+    obs = np.array([0.30, 0.95, 0.02])
+    model = Model(12 * u.micron)
+    dm = model.device_model
+    launch = Launch(block=256, lanes_per_ray=lanes, blocks_per_cu=0, variant=1)
+    before = dm.forward(v, obs, obs, launch=launch).cpu().numpy()
+    params = model.get_parameters()
+    params["comps"]["band3"]["delta_r"] = 1.6
+    model.update_parameters(params)
+    got = model.device_model.forward(v, obs, obs, launch=launch).cpu().numpy()
+    ipd = model_registry.get_model("dirbe")
+    comps = dict(ipd.comps)
+    comps[ComponentLabel.BAND3] = dataclasses.replace(comps[ComponentLabel.BAND3], delta_r=1.6)
+    want = OracleModel(12 * u.micron, model=dataclasses.replace(ipd, comps=comps)
+                       ).evaluate_vectors(v, obs, obs)
+    assert_parity(got, want)
+    band3 = list(ipd.comps).index(ComponentLabel.BAND3)
+    others = [c for c in range(got.shape[0]) if c != band3]
+    assert np.array_equal(got[others], before[others])
+    assert not np.array_equal(got[band3], before[band3])

@@ -133,8 +133,10 @@ struct ForwardOccupancy<LayoutKelsall4> {
 
 // SPEC: every component's frame path and form fixed at compile time to the
 // registry models' (RegistryProps); the host runs it only for blocks that meet
-// its preconditions (props_match).
-template <class LAYOUT, int BLOCK, int L, bool SPEC>
+// its preconditions (props_match). SHARE (with SPEC): the bands that share a
+// radial cut-off reuse it (kShareCut), for blocks whose delta_r values allow it
+// (the registry parameters; a fit that moves them runs SPEC alone).
+template <class LAYOUT, int BLOCK, int L, bool SPEC, bool SHARE = false>
 __global__ __launch_bounds__(BLOCK)
 __attribute__((amdgpu_waves_per_eu(ForwardOccupancy<LAYOUT>::value))) void forward_layout(
     FwdArgs a) {
@@ -148,7 +150,11 @@ __attribute__((amdgpu_waves_per_eu(ForwardOccupancy<LAYOUT>::value))) void forwa
   for (int64_t tile = blockIdx.x; tile < a.n_tiles; tile += gridDim.x) {
     const Ray y = load_ray<RAYS>(a, tile, local);
     double total = 0.0;
-    run_layout<L, SPEC ? RegistryProps<LAYOUT>::value : 0u>(LAYOUT{}, cx, a, y, total);
+    constexpr uint32_t kProps =
+        !SPEC ? 0u
+              : (SHARE ? RegistryProps<LAYOUT>::value
+                       : props_without_share(RegistryProps<LAYOUT>::value));
+    run_layout<L, kProps>(LAYOUT{}, cx, a, y, total);
     if (a.out_mode == ZODI_OUT_SUM && y.active && cx.sub == 0) a.out[y.r] = total;
   }
 }
@@ -315,12 +321,29 @@ FwdKernel pick_layout_block(int block, int L) {
   }
 }
 
-// The registry variant when the block meets its preconditions, else the one that
-// tests each component's flags.
+// The registry variant when the block meets its preconditions (with the shared
+// cut-offs for one lane per ray, the launch the tuner picks for large batches),
+// else the one that tests each component's flags.
 template <class LAYOUT>
 FwdKernel pick_layout(const zodi_model* m, int block, int L) {
-  if (props_match(m, RegistryProps<LAYOUT>::value))
-    return pick_layout_block<LAYOUT, true>(block, L);
+  constexpr uint32_t kProps = RegistryProps<LAYOUT>::value;
+  if constexpr (kProps != props_without_share(kProps)) {
+    if (L == 1 && props_match(m, kProps)) {
+      switch (block) {
+        case 64:
+          return forward_layout<LAYOUT, 64, 1, true, true>;
+        case 128:
+          return forward_layout<LAYOUT, 128, 1, true, true>;
+        case 256:
+          return forward_layout<LAYOUT, 256, 1, true, true>;
+        case 512:
+          return forward_layout<LAYOUT, 512, 1, true, true>;
+        default:
+          return nullptr;
+      }
+    }
+  }
+  if (props_match(m, props_without_share(kProps))) return pick_layout_block<LAYOUT, true>(block, L);
   return pick_layout_block<LAYOUT, false>(block, L);
 }
 
