@@ -358,6 +358,38 @@ def test_four_component_layout_matches_fd(gpu):
     sensitivity_check(g, fd, theta, 1e-6, 1e-6, "planck18 vs FD")
 
 
+@pytest.mark.parametrize("move_band3", [False, True], ids=["shared_cutoff", "moved_delta_r"])
+def test_planck_jacobian_shared_cutoff_matches_vjp(gpu, move_band3):
+    """planck18 at 857 GHz: the Jacobian pass of its registry variant forms the
+    radial cut-off of bands 1 and 3 (both delta_r = 1.5) once and keeps all four
+    components' sums in registers; with band 3's delta_r moved, as a fit moves it,
+    the variant without the sharing runs. Either way J^T w equals the weighted
+    pass's VJP (1e-11), I equals the forward kernel's (1e-13), and the VJP matches
+    central differences of the forward kernel (1e-6)."""
+    model = Model(857 * u.GHz, name="planck18")
+    v = healpix.pix2vec(8)
+    prob = LineOfSightProblem(model, v, OBS, EARTH)
+    theta = prob.space.theta0.copy()
+    k = prob.space.index("comps.band3.delta_r")
+    if move_band3:
+        theta[k] = 1.6
+    I, J = prob.jacobian(theta)
+    torch.testing.assert_close(I, prob.forward(theta), rtol=1e-13, atol=0)
+    w = torch.as_tensor(np.random.default_rng(12).normal(size=v.shape[1]), device="cuda")
+    g = prob.vjp(theta, w).cpu().numpy()
+    sensitivity_check((J @ w).cpu().numpy(), g, theta, 1e-11, 1e-11, "J^T w vs vjp")
+    pick = [prob.space.index(n) for n in ("comps.band1.delta_r", "comps.band3.delta_r",
+                                          "comps.band1.n_0", "comps.band3.n_0")]
+    fd = np.empty(len(pick))
+    for j, i in enumerate(pick):
+        h = 1e-6 * max(abs(theta[i]), 1e-3)
+        tp, tm = theta.copy(), theta.copy()
+        tp[i] += h
+        tm[i] -= h
+        fd[j] = (float((prob.forward(tp) * w).sum()) - float((prob.forward(tm) * w).sum())) / (2 * h)
+    sensitivity_check(g[pick], fd, theta[pick], 1e-6, 1e-6, "band cut-off partials vs FD")
+
+
 def test_empty_ray_set(gpu):
     model = Model(25 * u.micron)
     prob = LineOfSightProblem(model, np.zeros((3, 0)), OBS, EARTH)

@@ -42,12 +42,16 @@ namespace zodi {
 // The registry layouts' adjoint kernel. SPEC: the registry variant
 // (RegistryProps; the host checks props_match). NOSCAT: see adjoint_group. NREG:
 // components of a chunk whose accumulators stay in registers (the rest sum in
-// LDS).
+// LDS). SHARE (with SPEC, four-component passes): bands with a shared radial
+// cut-off (kShareCut) reuse it.
 template <class LAYOUT, int CHUNK, int SPLIT, bool JAC, bool SPEC = false, bool NOSCAT = false,
-          int NREG = reg_comps<JAC, CHUNK>()>
+          int NREG = reg_comps<JAC, CHUNK>(), bool SHARE = false>
 __global__ __launch_bounds__(kAdjBlock) __attribute__((amdgpu_waves_per_eu(2, 2))) void
 adjoint_layout(AdjArgs a) {
-  constexpr uint32_t PROPS = SPEC ? props_without_share(RegistryProps<LAYOUT>::value) : 0u;
+  constexpr uint32_t PROPS =
+      !SPEC ? 0u
+            : (SHARE ? RegistryProps<LAYOUT>::value
+                     : props_without_share(RegistryProps<LAYOUT>::value));
   // Components of a chunk beyond NREG: LDS accumulators, per wave kLdsSlots rows
   // of 64 lanes.
   constexpr int kLdsSlots = layout_lds_slots<CHUNK, NREG>(LAYOUT{});
@@ -215,21 +219,31 @@ AdjKernel pick_adj_split(int split, bool jac) {
 // Components of a four-component Jacobian chunk whose accumulators stay in
 // registers in the NOSCAT variants: without the scattering sums a third one fits
 // (250 VGPRs for the DIRBE layout, 237 for Kelsall4, no spill), which halves the
-// LDS adds per node; with all four in registers the kernel spills.
+// LDS adds per node; with all four in registers the kernel spills. With the
+// shared band cut-off as well (SHARE) all four fit (Kelsall4 in 256 VGPRs; DIRBE
+// spills 12, 64 bytes of scratch per lane), and no sum goes through LDS: 2.7 %
+// (DIRBE) and 5 % (Kelsall4) faster than three.
 constexpr int kJacRegCompsNoscat = 3;
+constexpr int kJacRegCompsShare = 4;
 
 // The registry variants (SPEC, and NOSCAT for Kelsall layouts) exist for the
 // default plans: four components per pass in the Jacobian pass, two in the
 // weighted pass.
 template <class LAYOUT>
-AdjKernel pick_adj_chunk(int chunk, int split, bool jac, bool spec, bool noscat) {
+AdjKernel pick_adj_chunk(int chunk, int split, bool jac, bool spec, bool share, bool noscat) {
   constexpr bool kKelsall = layout_integrand<LAYOUT>() == ZODI_INTEGRAND_KELSALL;
+  constexpr uint32_t kProps = RegistryProps<LAYOUT>::value;
   const bool default_plan = jac ? chunk == 4 : chunk == 2;
   if (spec && default_plan) {
     if (jac) {
       if (split != 1) return nullptr;
-      if (kKelsall && noscat)
-        return adjoint_layout<LAYOUT, 4, 1, true, true, kKelsall, kJacRegCompsNoscat>;
+      if constexpr (kKelsall && kProps != props_without_share(kProps)) {
+        if (noscat && share)
+          return adjoint_layout<LAYOUT, 4, 1, true, true, true, kJacRegCompsShare, true>;
+      }
+      if constexpr (kKelsall) {
+        if (noscat) return adjoint_layout<LAYOUT, 4, 1, true, true, true, kJacRegCompsNoscat>;
+      }
       return adjoint_layout<LAYOUT, 4, 1, true, true, false>;
     }
     if (kKelsall && noscat) return pick_adj_split<LAYOUT, 2, true, kKelsall>(split, false);
@@ -256,9 +270,9 @@ bool block_noscat(const zodi_model* m) {
 
 template <class LAYOUT>
 AdjKernel pick_adj_layout(const zodi_model* m, int chunk, int split, bool jac) {
-  return pick_adj_chunk<LAYOUT>(chunk, split, jac,
-                                props_match(m, props_without_share(RegistryProps<LAYOUT>::value)),
-                                block_noscat(m));
+  constexpr uint32_t kProps = RegistryProps<LAYOUT>::value;
+  return pick_adj_chunk<LAYOUT>(chunk, split, jac, props_match(m, props_without_share(kProps)),
+                                props_match(m, kProps), block_noscat(m));
 }
 
 AdjKernel pick_adjoint(const zodi_model* m, int chunk, int split, bool jac) {

@@ -252,18 +252,22 @@ __device__ __forceinline__ void adjoint_group(Grp<Ks...>, std::integer_sequence<
     }
     const double inv_R2 = nd.inv_R * nd.inv_R;
     double sw = 0.0;  // Kelsall: sum over the chunk of w n (1 - a) eps, for T_0 and delta
+    CutShare cs = {0.0, 0.0};  // a band's cut-off, for the bands that share it (in order)
     (
         [&] {
           constexpr int KIND = kinds[J];
           constexpr int NP = kind_nparams<KIND>();
           constexpr int G = geo_moments<is_feature(KIND)>();
+          constexpr int CUT = cut_mode<PROPS, J, Ks...>();
+          static_assert(CUT != kCutReuse || first_band<Ks...>() >= S,
+                        "a band that reuses a cut-off needs its publisher in the same pass");
           const ParamPtr P = Mk->comp[first + J];
           const uint32_t flags = Mk->comp_flags[first + J];
           constexpr uint32_t PC = (PROPS >> (4 * J)) & 15u;
           const CompFrame f =
               comp_frame<PC & 3>(P, flags, nd.X0, nd.X1, nd.X2, nd.R, nd.inv_R, nd.R2);
           const DensGrad<NP> dg =
-              density_grad<KIND, (PC & kFast) != 0>(P, flags, f, fdir[J - S], logR);
+              density_grad<KIND, (PC & kFast) != 0, CUT>(P, flags, f, fdir[J - S], logR, cs);
           auto& A = std::get<J - S>(acc);
           const double wn = wk * W[J - S];
           if constexpr (INTEGRAND == ZODI_INTEGRAND_KELSALL) {

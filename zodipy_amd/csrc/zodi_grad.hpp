@@ -84,27 +84,38 @@ __device__ __forceinline__ DensGrad<5> cloud_grad(ParamPtr P, uint32_t flags, co
   return o;
 }
 
+// A band's radial cut-off argument x = (R/delta_r)^20 and exp(-x) (0 where
+// 1 - exp(-x) rounds to 1), shared within a node by bands with the same cut-off.
+struct CutShare {
+  double x, ex;
+};
+
 // Band: n_0, delta_zeta [deg], v, p, delta_r. FAST (the registry kernels, for
 // blocks whose bands have p = 4): the band density's registry form (the plane
 // normal pre-divided by delta_zeta, the cut-off from R^2), p = 4 without a test.
-template <bool FAST = false>
-__device__ __forceinline__ DensGrad<5> band_grad(ParamPtr P, uint32_t flags, const CompFrame& f) {
+// CUT (BandCut): kCutPublish stores x and exp(-x) in `cs`, kCutReuse takes them
+// from it (a band whose cut-off the host has checked to equal the publisher's).
+template <bool FAST = false, int CUT = kCutOwn>
+__device__ __forceinline__ DensGrad<5> band_grad(ParamPtr P, uint32_t flags, const CompFrame& f,
+                                                 CutShare& cs) {
   const double three_n0 = P[6], dzeta = P[7], p = P[8], v = P[9], dr = P[10];
   const double inv_dzeta = P[13], inv_v = P[14], inv_dr = P[15];
   const bool p4 = FAST || (flags & ZODI_CF_P_IS_4);
-  double zd, zsign, x;
+  double zd, zsign, x = 0.0;
   if constexpr (FAST) {
     const double zs = f.x * P[ZODI_P_D0 + 3] - f.y * P[ZODI_P_D0 + 4] + f.z * P[ZODI_P_D0 + 5];
     zd = fabs(zs) * f.inv_r;
     zsign = sign_of(zs);  // the sign of Z
-    const double y = f.r2 * P[ZODI_P_D0 + 6];
-    const double y2 = y * y;
-    const double y4 = y2 * y2;
-    x = (y4 * y4) * y2;
+    if constexpr (CUT != kCutReuse) {
+      const double y = f.r2 * P[ZODI_P_D0 + 6];
+      const double y2 = y * y;
+      const double y4 = y2 * y2;
+      x = (y4 * y4) * y2;
+    }
   } else {
     zd = (fabs(f.zc) * f.inv_r) * inv_dzeta;
     zsign = sign_of(f.zc);
-    x = pow20(f.r * inv_dr);
+    if constexpr (CUT != kCutReuse) x = pow20(f.r * inv_dr);
   }
   const double zd2 = zd * zd;
   const double zd4 = zd2 * zd2;
@@ -117,7 +128,14 @@ __device__ __forceinline__ DensGrad<5> band_grad(ParamPtr P, uint32_t flags, con
   const double t1 = three_n0 * f.inv_r;
   const double t2 = exp_fast(-zd6);
   const double t3 = 1.0 + zdp * inv_v;
-  const double ex = band_cutoff_exp(x);  // 0 where 1 - exp(-x) == 1
+  double ex;
+  if constexpr (CUT == kCutReuse) {
+    x = cs.x;
+    ex = cs.ex;
+  } else {
+    ex = band_cutoff_exp(x);  // 0 where 1 - exp(-x) == 1
+    if constexpr (CUT == kCutPublish) cs = {x, ex};
+  }
   const double t4 = 1.0 - ex;
   DensGrad<5> o;
   const double t24 = t2 * t4;
@@ -346,16 +364,16 @@ __device__ __forceinline__ DensGrad<6> broad_band_grad(ParamPtr P, const CompFra
 
 // Value and parameter partials of a component whose kind is known at compile time.
 // `logR` = log |X| (heliocentric), reused where the component frame is the Sun's.
-// FAST: the registry forms (band_grad).
-template <int KIND, bool FAST = false>
+// FAST: the registry forms (band_grad); CUT and `cs`: a band's shared cut-off.
+template <int KIND, bool FAST = false, int CUT = kCutOwn>
 __device__ __forceinline__ DensGrad<kind_nparams<KIND>()> density_grad(ParamPtr P, uint32_t flags,
                                                                         const CompFrame& f,
                                                                         const FeatureDir& fd,
-                                                                        double logR) {
+                                                                        double logR, CutShare& cs) {
   if constexpr (KIND == ZODI_CLOUD) {
     return cloud_grad(P, flags, f, logR);
   } else if constexpr (KIND == ZODI_BAND) {
-    return band_grad<FAST>(P, flags, f);
+    return band_grad<FAST, CUT>(P, flags, f, cs);
   } else if constexpr (KIND == ZODI_RING) {
     return ring_grad(P, f);
   } else if constexpr (KIND == ZODI_FEATURE) {
