@@ -228,7 +228,9 @@ constexpr int kJacRegCompsShare = 4;
 
 // The registry variants (SPEC, and NOSCAT for Kelsall layouts) exist for the
 // default plans: four components per pass in the Jacobian pass, two in the
-// weighted pass.
+// weighted pass; SHARE (with NOSCAT) for the Jacobian pass of the layouts whose
+// registry properties share a band cut-off. `spec` and `share` say whether the
+// block meets the preconditions without and with the sharing.
 template <class LAYOUT>
 AdjKernel pick_adj_chunk(int chunk, int split, bool jac, bool spec, bool share, bool noscat) {
   constexpr bool kKelsall = layout_integrand<LAYOUT>() == ZODI_INTEGRAND_KELSALL;
