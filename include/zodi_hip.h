@@ -315,8 +315,9 @@ enum {
  * block must be 0 or 256, lanes_per_ray 0 (chosen from n_rays), 1, 2 or 4 (lanes
  * sharing one ray's nodes), blocks_per_cu 0..64 (0 = 2), and variant the
  * components evaluated per pass over a group's nodes: 0 (default: 4 in the
- * Jacobian pass, 2 otherwise), 1, 2 or 4 (four in the Jacobian pass, the third
- * and fourth with their sums in LDS; two in the weighted pass).
+ * Jacobian pass, 2 otherwise), 1, 2 or 4 (four in the Jacobian pass, the sums of
+ * up to two of them in LDS where registers do not hold all four; two in the
+ * weighted pass).
  * Every component layout the block can describe is supported: the registry
  * layouts (the Kelsall cloud and three bands, optionally followed by the ring
  * and the feature; the RRM model in its registry grouping or one component per

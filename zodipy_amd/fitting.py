@@ -175,7 +175,7 @@ class LineOfSightProblem:
         self._r_obs = r_obs.cpu().numpy()
         self._beyond_cache = {self._group_key(base.packed.block): beyond}
         # Adjoint launch (None = library defaults); ``variant`` = components per pass
-        # (1, 2, or 4: the Jacobian pass's default, two of them summing in LDS).
+        # (1, 2, or 4: the Jacobian pass's default, up to two of them summing in LDS).
         self.launch: Launch | None = None
         self.store_jacobian = bool(store_jacobian)
         self._theta_key: bytes | None = None
