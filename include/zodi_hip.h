@@ -27,6 +27,9 @@
  *                         parameters), for parameter fitting
  *   zodi_jacobian         no reference counterpart: the per-ray Jacobian of the
  *                         same map, for Gauss-Newton and Fisher-matrix work
+ *   zodi_table_vjp        no reference counterpart: the gradient with respect to
+ *                         the Planck table (zodipy/blackbody.py:33-49), for fits of
+ *                         the bandpass weights
  *   zodi_spectral_setup   zodipy/unpack_model.py:23-173 and zodipy/blackbody.py:33-49
  *                         (effective spectral parameters and the Planck table,
  *                         written into a device model block)
@@ -338,6 +341,34 @@ zodi_status zodi_adjoint(const zodi_model* model_host, const zodi_model* model_d
                          double inv_var_scalar, double* model_out, double* grad,
                          double* workspace, int64_t workspace_bytes, const zodi_launch* launch,
                          void* stream);
+
+/*
+ * zodi_table_vjp — gradient with respect to the block's Planck table B_k =
+ * B(T_k), the bandpass-weighted blackbody emission that zodipy/blackbody.py:33-49
+ * tabulates and zodipy/brightness.py:48 interpolates (np.interp at each node's
+ * temperature). No reference counterpart (the reference has no derivatives):
+ * with the effective-parameter gradient of zodi_adjoint it gives the gradient
+ * with respect to the bandpass weights, in which both are linear. Arguments and
+ * modes as in zodi_adjoint's recompute path; grad_table [ZODI_N_TEMPS + 1]
+ * receives sum_r sum_c g_{c,r} dI_{c,r}/dB_k for k < n_temps (0 beyond), then
+ * chi^2 (ZODI_ADJ_CHI2; else 0). A node between entries j and j + 1 at fraction f
+ * contributes (1 - f) and f; a node at or past the last entry, or clamped below
+ * the first, all of it to that entry. workspace: zodi_table_vjp_workspace bytes
+ * (block partials, then I unless model_out is given). launch: block,
+ * lanes_per_ray and blocks_per_cu as in zodi_adjoint; variant is ignored. Any component layout. The reduction order is fixed by the launch
+ * shape; within a wave, lanes that meet in one entry add in hardware order.
+ */
+int64_t zodi_table_vjp_workspace(const zodi_model* model_host, int64_t n_rays,
+                                 const zodi_launch* launch);
+
+zodi_status zodi_table_vjp(const zodi_model* model_host, const zodi_model* model_dev,
+                           const double* u, int64_t n_rays, const int64_t* seg_offsets,
+                           int32_t n_seg, const double* obs, const double* earth,
+                           int32_t obs_mode, const uint32_t* beyond, int32_t mode,
+                           const double* weights, const double* target, const double* inv_var,
+                           double inv_var_scalar, double* model_out, double* grad_table,
+                           double* workspace, int64_t workspace_bytes, const zodi_launch* launch,
+                           void* stream);
 
 /*
  * zodi_adjoint_noalloc — zodi_adjoint without a workspace: no reference

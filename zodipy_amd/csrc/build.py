@@ -20,7 +20,7 @@ ROOT = PKG.parent
 INCLUDE = ROOT / "include"
 LIB = PKG / "libzodi_hip.so"
 SOURCES = [CSRC / "zodi_forward.hip", CSRC / "zodi_adjoint.hip", CSRC / "zodi_adjoint_generic.hip",
-           CSRC / "zodi_pointing.hip", CSRC / "zodi_spectral.hip"]
+           CSRC / "zodi_table_adjoint.hip", CSRC / "zodi_pointing.hip", CSRC / "zodi_spectral.hip"]
 HEADERS = [
     CSRC / "zodi_adjoint.hpp",
     CSRC / "zodi_common.hpp",

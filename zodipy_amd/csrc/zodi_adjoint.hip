@@ -408,6 +408,18 @@ zodi_status adjoint_plan(const zodi_launch* launch, int64_t n_rays, bool jac, Ad
   return ZODI_OK;
 }
 
+// Rows of the Planck-table gradient: one entry per table temperature, then chi^2.
+constexpr int kTableRow = ZODI_N_TEMPS + 1;
+
+// Launch plan of the table adjoint: the weighted pass's (two resident blocks per
+// CU by default; the kernel is compiled for two waves per SIMD); the
+// components-per-pass variant does not apply.
+zodi_status table_plan(const zodi_launch* launch, int64_t n_rays, AdjPlan& p) {
+  zodi_launch l = launch != nullptr ? *launch : zodi_launch{0, 0, 0, 0};
+  l.variant = 0;
+  return adjoint_plan(&l, n_rays, false, p);
+}
+
 // Largest Jacobian the chi^2 mode stores in its workspace (a quarter of one
 // MI355X's 288 GB of HBM); larger problems recompute the integrand instead.
 constexpr int64_t kJacobianMaxBytes = int64_t(72) << 30;
@@ -685,6 +697,53 @@ zodi_status zodi_adjoint(const zodi_model* model_host, const zodi_model* model_d
   const RayArgs r = {u, n_rays, seg_offsets, n_seg, obs, earth, obs_mode, beyond};
   return adjoint_recompute(model_host, model_dev, r, mode, weights, target, inv_var,
                            inv_var_scalar, plan, k, workspace, vals, grad, stream);
+}
+
+int64_t zodi_table_vjp_workspace(const zodi_model* model_host, int64_t n_rays,
+                                 const zodi_launch* launch) {
+  if (model_host == nullptr || validate_model(model_host) != ZODI_OK || n_rays < 0) return -1;
+  AdjPlan p;
+  if (table_plan(launch, n_rays, p) != ZODI_OK) return -1;
+  return recompute_bytes(p, kTableRow, n_rays);
+}
+
+zodi_status zodi_table_vjp(const zodi_model* model_host, const zodi_model* model_dev,
+                           const double* u, int64_t n_rays, const int64_t* seg_offsets,
+                           int32_t n_seg, const double* obs, const double* earth,
+                           int32_t obs_mode, const uint32_t* beyond, int32_t mode,
+                           const double* weights, const double* target, const double* inv_var,
+                           double inv_var_scalar, double* model_out, double* grad_table,
+                           double* workspace, int64_t workspace_bytes, const zodi_launch* launch,
+                           void* stream) {
+  zodi_status st = check_rays(model_host, model_dev, u, n_rays, seg_offsets, n_seg, obs, earth,
+                              obs_mode, beyond);
+  if (st != ZODI_OK) return st;
+  if (grad_table == nullptr || workspace == nullptr)
+    return fail(ZODI_ERR_ARG, "null pointer (grad_table or workspace)");
+  if (mode == ZODI_ADJ_CHI2) {
+    if (n_rays > 0 && target == nullptr) return fail(ZODI_ERR_ARG, "chi2 mode needs target");
+  } else if (mode == ZODI_ADJ_VJP_SUM || mode == ZODI_ADJ_VJP_COMPS) {
+    if (n_rays > 0 && weights == nullptr) return fail(ZODI_ERR_ARG, "vjp mode needs weights");
+  } else {
+    return fail(ZODI_ERR_ARG, "unknown adjoint mode %d", mode);
+  }
+  AdjPlan plan;
+  st = table_plan(launch, n_rays, plan);
+  if (st != ZODI_OK) return st;
+  AdjKernel k = table_adjoint_kernel(model_host->integrand, plan.split);
+  if (k == nullptr)
+    return fail(ZODI_ERR_ARG, "no table adjoint kernel for %d lanes per ray", plan.split);
+  const int64_t need = recompute_bytes(plan, kTableRow, n_rays);
+  if (workspace_bytes < need)
+    return fail(ZODI_ERR_ARG, "workspace of %lld bytes needed, %lld given", (long long)need,
+                (long long)workspace_bytes);
+  double* vals = model_out != nullptr ? model_out : workspace + plan.grid * kTableRow;
+  const RayArgs r = {u, n_rays, seg_offsets, n_seg, obs, earth, obs_mode, beyond};
+  // the table's entries take the place of the parameters: rows of ZODI_N_TEMPS + 1
+  zodi_model shape = *model_host;
+  shape.n_params = ZODI_N_TEMPS;
+  return adjoint_recompute(&shape, model_dev, r, mode, weights, target, inv_var, inv_var_scalar,
+                           plan, k, workspace, vals, grad_table, stream);
 }
 
 zodi_status zodi_adjoint_noalloc(const zodi_model* model_host, const zodi_model* model_dev,

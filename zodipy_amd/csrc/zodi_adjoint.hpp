@@ -535,8 +535,10 @@ __device__ __forceinline__ void adjoint_body(const AdjArgs& a, RAY ray) {
   }
 }
 
-// Kernel entry points of the other adjoint source file (custom component layouts).
+// Kernel entry points of the other adjoint source files: custom component
+// layouts, and the gradient with respect to the Planck table.
 using AdjKernel = void (*)(AdjArgs);
 AdjKernel generic_adjoint_kernel(int integrand, int split, bool jac);
+AdjKernel table_adjoint_kernel(int integrand, int split);
 
 }  // namespace zodi

@@ -52,6 +52,7 @@ from zodipy_amd._lib import (
     Launch,
 )
 from zodipy_amd.packing import (
+    N_TEMPS,
     PackedModel,
     beyond_bits_for,
     pack_model,
@@ -61,8 +62,8 @@ from zodipy_amd.packing import (
 from zodipy_amd.runtime import DeviceModel, _f64_rows, _stream
 from zodipy_amd.zodiacal_light_model import Integrand
 
-__all__ = ["ParameterSpace", "LineOfSightProblem", "SpectralTableFit", "emission", "chi2",
-           "chi2_tables"]
+__all__ = ["ParameterSpace", "LineOfSightProblem", "SpectralTableFit", "BandpassFit", "emission",
+           "chi2", "chi2_tables", "chi2_bandpass"]
 
 
 class ParameterSpace:
@@ -339,6 +340,58 @@ class LineOfSightProblem:
             raise ValueError(msg)
         return t.contiguous()
 
+    def _table_adjoint(self, theta, mode: int, weights=None, target=None, inv_var=None,
+                       inv_var_scalar: float = 1.0, blocks=None) -> torch.Tensor:
+        host, dev = blocks if blocks is not None else self._model(theta)[:2]
+        need = int(self.lib.zodi_table_vjp_workspace(_lib.host_ptr(host), self.n_rays,
+                                                     self.launch))
+        if need < 0:
+            err = self.lib.zodi_last_error().decode(errors="replace")
+            msg = f"zodi_table_vjp_workspace rejected the model or launch: {err}"
+            raise RuntimeError(msg)
+        ws = torch.empty(max((need + 7) // 8, 1), dtype=torch.float64, device=self.device)
+        grad = torch.empty(N_TEMPS + 1, dtype=torch.float64, device=self.device)
+
+        def ptr(t):
+            return t.data_ptr() if t is not None else None
+
+        st = self.lib.zodi_table_vjp(
+            _lib.host_ptr(host), dev.data_ptr(), *self._ptr_args(host), mode, ptr(weights),
+            ptr(target), ptr(inv_var), float(inv_var_scalar), None, grad.data_ptr(),
+            ws.data_ptr(), need, self.launch, _stream())
+        _lib.check(st, "zodi_table_vjp")
+        return grad
+
+    def table_vjp(self, theta, weights, return_comps: bool = False, blocks=None) -> torch.Tensor:
+        """sum_r (sum_c) weights * dI/dB_k over the Planck table entries B_k (100,).
+
+        B_k = B(T_k) is the bandpass-weighted blackbody emission the model block
+        tabulates (zodipy/blackbody.py:33-49); entries past the block's table are 0.
+        """
+        if return_comps:
+            w = self._rows(weights, (self.n_comps, self.n_rays), "weights")
+            return self._table_adjoint(theta, ADJ_VJP_COMPS, weights=w, blocks=blocks)[:-1]
+        w = self._rows(weights, (self.n_rays,), "weights")
+        return self._table_adjoint(theta, ADJ_VJP_SUM, weights=w, blocks=blocks)[:-1]
+
+    def chi2_table_grad(self, theta, data, inv_var=None, blocks=None):
+        """(chi^2, d chi^2 / dB_k (100,)) for the Planck table entries, on the device."""
+        d = self._rows(data, (self.n_rays,), "data")
+        iv_t, iv_s = self._inv_var(inv_var)
+        g = self._table_adjoint(theta, ADJ_CHI2, target=d, inv_var=iv_t, inv_var_scalar=iv_s,
+                                blocks=blocks)
+        return g[-1], g[:-1]
+
+    def _inv_var(self, inv_var):
+        """(per-ray tensor or None, scalar) from None, a scalar or an (n,) array."""
+        if inv_var is None:
+            return None, 1.0
+        if np.ndim(inv_var) == 0 and not isinstance(inv_var, torch.Tensor):
+            return None, float(inv_var)
+        if isinstance(inv_var, torch.Tensor) and inv_var.ndim == 0:
+            return None, float(inv_var)
+        return self._rows(inv_var, (self.n_rays,), "inv_var"), 1.0
+
     def vjp(self, theta, weights, return_comps: bool = False) -> torch.Tensor:
         """sum_r (sum_c) weights * dI/dtheta for weights of the forward output's shape."""
         if return_comps:
@@ -356,14 +409,7 @@ class LineOfSightProblem:
         in ``forward``.
         """
         d = self._rows(data, (self.n_rays,), "data")
-        iv_t, iv_s = None, 1.0
-        if inv_var is not None:
-            if np.ndim(inv_var) == 0 and not isinstance(inv_var, torch.Tensor):
-                iv_s = float(inv_var)
-            elif isinstance(inv_var, torch.Tensor) and inv_var.ndim == 0:
-                iv_s = float(inv_var)
-            else:
-                iv_t = self._rows(inv_var, (self.n_rays,), "inv_var")
+        iv_t, iv_s = self._inv_var(inv_var)
         model = (torch.empty(self.n_rays, dtype=torch.float64, device=self.device)
                  if return_model else None)
         g = self._adjoint(theta, ADJ_CHI2, target=d, inv_var=iv_t, inv_var_scalar=iv_s,
@@ -457,6 +503,121 @@ class SpectralTableFit:
         return c2, g.reshape(-1)
 
 
+class BandpassFit:
+    """chi^2 fits of a model's bandpass weights at its fixed frequencies, on the device.
+
+    The model normalises the weights w by their trapezoid over x
+    (zodipy/model.py:82-93): wn = w / (c . w), c the trapezoid coefficients. The
+    effective spectral parameters (zodipy/unpack_model.py:140-173) and the Planck
+    table (zodipy/blackbody.py:33-49) are then linear in wn: eff = K @ wn, K's
+    column i the unpacking of a unit weight at x_i (every interpolation,
+    extrapolation and unit rule of the unpacking itself), and
+    B_k = sum_i c_i wn_i B_nu_i(T_k). Per call ``zodi_spectral_setup`` writes both
+    into the fit's own device block; chi^2 and d chi^2/d eff come from
+    ``zodi_adjoint``, d chi^2/dB_k from ``zodi_table_vjp``, and the chain rule,
+    through the normalisation, runs on the device:
+
+        d chi^2/d wn_i = sum_t K[t, i] d chi^2/d eff_t + c_i sum_k B_nu_i(T_k) d chi^2/dB_k,
+        d chi^2/d w_j = (d chi^2/d wn_j - c_j sum_i wn_i d chi^2/d wn_i) / (c . w).
+
+    Everything else stays at ``theta`` (default theta0). The gradient is
+    orthogonal to w (the normalisation removes the scale).
+    """
+
+    def __init__(self, problem: LineOfSightProblem, theta=None) -> None:
+        from zodipy_amd import units as u
+        from zodipy_amd.spectral import planck_table, spectral_tables, unpack_spectral
+
+        self.problem = problem
+        space = problem.space
+        if space._normalized_weights is None or np.size(space._x.value) < 2:
+            msg = "the model has no bandpass (weights over two or more frequencies)"
+            raise ValueError(msg)
+        self.theta = space.theta0.copy() if theta is None else space._check(theta)
+        x = space._x
+        xv = np.asarray(x.value, dtype=np.float64)
+        self.n_freq = int(xv.size)
+        d = np.diff(xv)
+        c = np.zeros_like(xv)
+        c[:-1] += d / 2.0
+        c[1:] += d / 2.0
+        self.weights0 = np.asarray(space._normalized_weights, dtype=np.float64).copy()
+        tables = spectral_tables(space._ipd, x, self.weights0, space._extrapolate)
+        self.names = tuple(t.name for t in tables)
+        eye = np.eye(self.n_freq)
+        K = np.zeros((len(tables), self.n_freq))
+        for i in range(self.n_freq):
+            sp = unpack_spectral(space._ipd, x, eye[i], space._extrapolate)
+            for t, tab in enumerate(tables):
+                K[t, i] = _effective_value(sp, tab.name)
+        bnu = np.stack([planck_table(x[i:i + 1], None)[1] for i in range(self.n_freq)])
+        dev = problem.device
+        self._c = torch.as_tensor(c, device=dev)
+        self._K = torch.as_tensor(K, device=dev)
+        self._bnu = torch.as_tensor(bnu, device=dev)  # (n_freq, 100)
+        self._nu = torch.as_tensor(
+            np.asarray(x.to_value(u.Hz, equivalencies=u.spectral()), dtype=np.float64), device=dev)
+        self._targets = np.array([t.target for t in tables], dtype=np.int32)
+        self._eff_index = torch.as_tensor([space.index(t.name) for t in tables], device=dev,
+                                          dtype=torch.int64)
+        scatter = tuple(lab for lab in space._ipd.comps
+                        if any(t.name == f"albedo.{lab.value}" for t in tables))
+        packed = space.pack(self.theta, force_scatter=scatter)
+        self._host = np.ascontiguousarray(packed.block)
+        self._dev = torch.from_numpy(self._host.view(np.uint8).copy()).to(dev)
+
+    def _normalised(self, weights) -> tuple[torch.Tensor, torch.Tensor]:
+        w = torch.as_tensor(weights, dtype=torch.float64, device=self.problem.device)
+        if w.shape != (self.n_freq,):
+            msg = f"weights must have shape ({self.n_freq},), got {tuple(w.shape)}"
+            raise ValueError(msg)
+        S = torch.dot(self._c, w)
+        return w, w / S
+
+    def set_weights(self, weights) -> None:
+        """Write the effective values and the Planck table of ``weights`` into the device block."""
+        _, wn = self._normalised(weights)
+        n_tab = len(self.names)
+        tabs = wn.expand(max(n_tab, 1), -1).contiguous()
+        q = (self._c * wn).contiguous()
+        st = self.problem.lib.zodi_spectral_setup(
+            _lib.host_ptr(self._host), self._dev.data_ptr(), n_tab, self.n_freq,
+            self._K.data_ptr() if n_tab else None, tabs.data_ptr() if n_tab else None,
+            self._targets.ctypes.data if n_tab else None, self.n_freq, self._nu.data_ptr(),
+            q.data_ptr(), _stream())
+        _lib.check(st, "zodi_spectral_setup")
+
+    def forward(self, weights, return_comps: bool = False) -> torch.Tensor:
+        self.set_weights(weights)
+        return self.problem.forward(None, return_comps, blocks=(self._host, self._dev))
+
+    def chi2_and_grad(self, weights, data, inv_var=None):
+        """(chi^2, d chi^2 / d weights (n_freq,)) on the device."""
+        w, wn = self._normalised(weights)
+        self.set_weights(w)
+        blocks = (self._host, self._dev)
+        c2, g = self.problem.chi2_and_grad(None, data, inv_var, blocks=blocks)
+        _, g_table = self.problem.chi2_table_grad(None, data, inv_var, blocks=blocks)
+        d_wn = self._c * (self._bnu @ g_table)
+        if len(self.names):
+            d_wn = d_wn + self._K.t() @ g[self._eff_index]
+        S = torch.dot(self._c, w)
+        d_w = (d_wn - self._c * torch.dot(wn, d_wn)) / S
+        return c2, d_w
+
+
+def _effective_value(spectral: dict, name: str) -> float:
+    """The effective value named as in the gradient vector from an unpacked dict."""
+    if "." in name:
+        key, label = name.split(".", 1)
+        for lab, vals in spectral["comp"].items():
+            if lab.value == label:
+                return float(vals[key])
+        msg = f"no component {label!r}"
+        raise KeyError(msg)
+    return float(spectral["shared"][name])
+
+
 class _Emission(torch.autograd.Function):
     @staticmethod
     def forward(ctx, theta, problem, return_comps):
@@ -519,3 +680,8 @@ class _Chi2Tables(torch.autograd.Function):
 def chi2_tables(values: torch.Tensor, fit: SpectralTableFit, data, inv_var=None) -> torch.Tensor:
     """Differentiable chi^2 as a function of the spectral table entries."""
     return _Chi2Tables.apply(values, fit, data, inv_var)
+
+
+def chi2_bandpass(weights: torch.Tensor, fit: BandpassFit, data, inv_var=None) -> torch.Tensor:
+    """Differentiable chi^2 as a function of the bandpass weights."""
+    return _Chi2Tables.apply(weights, fit, data, inv_var)
