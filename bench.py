@@ -334,13 +334,16 @@ def grad_cost(dev, reps: int = 10) -> dict:
         chi^2 call over the whole batch against the batched forward; the
         registry's four-component planck18 is timed the same way. The device
         set-up of that bandpass's Planck table (30 frequencies x 100
-        temperatures) and effective spectral values is timed on its own.
+        temperatures) and effective spectral values is timed on its own, and
+        so is chi^2 with its gradient with respect to the 30 bandpass weights
+        (BandpassFit: set-up, chi^2 and the parameter gradient, the Planck-table
+        gradient, chain rule) on the six-component batch.
     """
     import torch
 
     from zodipy_amd import Model, healpix
     from zodipy_amd import units as zu
-    from zodipy_amd.fitting import LineOfSightProblem, SpectralTableFit
+    from zodipy_amd.fitting import BandpassFit, LineOfSightProblem, SpectralTableFit
     from zodipy_amd.zodiacal_light_model import model_registry
 
     out = {}
@@ -431,6 +434,18 @@ def grad_cost(dev, reps: int = 10) -> dict:
             log(f"spectral set-up on the device (30-frequency bandpass, {len(sfit.tables)} "
                 f"tables): {t_set * 1e3:.1f} us")
             del sfit
+            bfit = BandpassFit(bprob)
+            w = torch.as_tensor(bfit.weights0, device=dev)
+            bfit.chi2_and_grad(w, bdata)
+            t_bp = _event_time(lambda: bfit.chi2_and_grad(w, bdata), max(2, reps // 2))
+            t_tv = _event_time(lambda: bprob.chi2_table_grad(th, bdata), max(2, reps // 2))
+            out["bandpass_weights_857GHz_tophat30"] = {
+                "n_freq": 30, "chi2_grad_ms": round(t_bp, 3), "ratio_vs_fwd": round(t_bp / bt_fwd, 4),
+                "table_vjp_ms": round(t_tv, 3)}
+            log(f"chi2 gradient w.r.t. the 30 bandpass weights ({mname}): {t_bp:.2f} ms "
+                f"(set-up, chi2 + parameter gradient, Planck-table gradient {t_tv:.2f} ms, chain "
+                f"rule), ratio {t_bp / bt_fwd:.3f}")
+            del bfit
         del bprob
     out["ratio"] = out["chi2_dirbe25_nside128"]["ratio"]
     return out

@@ -411,11 +411,14 @@ zodi_status adjoint_plan(const zodi_launch* launch, int64_t n_rays, bool jac, Ad
 // Rows of the Planck-table gradient: one entry per table temperature, then chi^2.
 constexpr int kTableRow = ZODI_N_TEMPS + 1;
 
-// Launch plan of the table adjoint: the weighted pass's (two resident blocks per
-// CU by default; the kernel is compiled for two waves per SIMD); the
+// Launch plan of the table adjoint: the weighted pass's, with as many resident
+// blocks per CU as its kernel is compiled for (four for the registry layouts, two
+// for the kind switch) unless the caller sets blocks_per_cu; the
 // components-per-pass variant does not apply.
-zodi_status table_plan(const zodi_launch* launch, int64_t n_rays, AdjPlan& p) {
+zodi_status table_plan(const zodi_model* m, const zodi_launch* launch, int64_t n_rays,
+                       AdjPlan& p) {
   zodi_launch l = launch != nullptr ? *launch : zodi_launch{0, 0, 0, 0};
+  if (l.blocks_per_cu == 0) l.blocks_per_cu = table_adjoint_is_layout(m) ? 4 : 2;
   l.variant = 0;
   return adjoint_plan(&l, n_rays, false, p);
 }
@@ -703,7 +706,7 @@ int64_t zodi_table_vjp_workspace(const zodi_model* model_host, int64_t n_rays,
                                  const zodi_launch* launch) {
   if (model_host == nullptr || validate_model(model_host) != ZODI_OK || n_rays < 0) return -1;
   AdjPlan p;
-  if (table_plan(launch, n_rays, p) != ZODI_OK) return -1;
+  if (table_plan(model_host, launch, n_rays, p) != ZODI_OK) return -1;
   return recompute_bytes(p, kTableRow, n_rays);
 }
 
@@ -728,9 +731,9 @@ zodi_status zodi_table_vjp(const zodi_model* model_host, const zodi_model* model
     return fail(ZODI_ERR_ARG, "unknown adjoint mode %d", mode);
   }
   AdjPlan plan;
-  st = table_plan(launch, n_rays, plan);
+  st = table_plan(model_host, launch, n_rays, plan);
   if (st != ZODI_OK) return st;
-  AdjKernel k = table_adjoint_kernel(model_host->integrand, plan.split);
+  AdjKernel k = table_adjoint_kernel(model_host, plan.split);
   if (k == nullptr)
     return fail(ZODI_ERR_ARG, "no table adjoint kernel for %d lanes per ray", plan.split);
   const int64_t need = recompute_bytes(plan, kTableRow, n_rays);

@@ -539,6 +539,7 @@ __device__ __forceinline__ void adjoint_body(const AdjArgs& a, RAY ray) {
 // layouts, and the gradient with respect to the Planck table.
 using AdjKernel = void (*)(AdjArgs);
 AdjKernel generic_adjoint_kernel(int integrand, int split, bool jac);
-AdjKernel table_adjoint_kernel(int integrand, int split);
+AdjKernel table_adjoint_kernel(const zodi_model* m, int split);
+bool table_adjoint_is_layout(const zodi_model* m);  // the kernel is a registry layout's
 
 }  // namespace zodi

@@ -16,13 +16,16 @@
 // A bandpass fit needs this and the effective-parameter gradient of the other
 // adjoint kernels: both are linear in the normalised bandpass weights.
 //
-// One ray per lane (or SPLIT lanes sharing its nodes); a group's components are
-// evaluated through the kind switch (any layout). Each node adds its two shares
+// One ray per lane (or SPLIT lanes sharing its nodes). The registry layouts run
+// with their kinds fixed at compile time (table_layout), any other layout through
+// the kind switch (table_adjoint). Each node adds its two shares
 // to the wave's row in LDS with wavefront-scope atomic adds (lanes of a wave may
 // meet in one entry); the block adds its wave rows in wave order and a reduction
 // kernel adds the blocks in block order.
 
 #include <hip/hip_runtime.h>
+
+#include <utility>
 
 #include "zodi_adjoint.hpp"
 
@@ -100,6 +103,104 @@ __device__ __forceinline__ void table_ray(const Ctx& cx, const AdjArgs& a, const
   }
 }
 
+// Registry layouts: the group's kinds (and, with PROPS, the registry forms of the
+// forward kernels) fixed at compile time, the densities inlined.
+template <int INTEGRAND, int SPLIT, uint32_t PROPS, int... Ks, int... J>
+__device__ __forceinline__ void table_group(Grp<Ks...>, std::integer_sequence<int, J...>,
+                                            const Ctx& cx, const AdjArgs& a, const Ray& y, int g,
+                                            int first, double gscale, double* row, int sub) {
+  constexpr int C = sizeof...(Ks);
+  constexpr int kinds[C] = {Ks...};
+  const ModelPtr M = cx.M;
+  const int n_last = M->n_temps - 1;
+  double h, mid;
+  group_range(M, g, y, h, mid);
+  const double us = M->group_T0[g] * cx.tb.inv_step;
+  const double uo = cx.tb.T_first * cx.tb.inv_step;
+  const double mdelta = -M->group_delta[g];
+  FeatureDir fdir[C];
+  ((fdir[J] = is_feature(kinds[J]) ? feature_dir(M->comp[first + J], y.ex, y.ey)
+                                   : FeatureDir{1.0, 0.0}),
+   ...);
+  double cw[C];
+  (
+      [&] {
+        const int c = first + J;
+        const double s =
+            INTEGRAND == ZODI_INTEGRAND_KELSALL ? M->comp[c][ZODI_P_SCALE] : cx.calibration;
+        double w = gscale;
+        if (a.mode == ZODI_ADJ_VJP_COMPS)
+          w = y.active ? a.weights[(int64_t)c * a.f.n_rays + y.r] : 0.0;
+        cw[J] = (h * s) * w;
+      }(),
+      ...);
+  const int K = y.active ? cx.K : 0;
+  for (int k = sub; k < K; k += SPLIT) {
+    const ModelPtr Mk = opaque(M);
+    const double xk = Mk->gl_x[k];
+    const double wk = Mk->gl_w[k];
+    const double s_los = h * xk + mid;
+    Node nd;
+    nd.X0 = s_los * y.ux + y.ox;
+    nd.X1 = s_los * y.uy + y.oy;
+    nd.X2 = s_los * y.uz + y.oz;
+    nd.R2 = nd.X0 * nd.X0 + nd.X1 * nd.X1 + nd.X2 * nd.X2;
+    norm_and_inverse_node(nd.R2, nd.R, nd.inv_R);
+    nd.B = 0.0;
+    nd.phi = 0.0;
+    const double u = fma(pow_radius(nd.R, mdelta), us, -uo);
+    const double uc = fmin(fmax(u, 0.0), (double)(ZODI_N_TEMPS - 1));
+    const int jt = (int)uc;
+    const double f = __builtin_amdgcn_fract(uc);
+    double cut = 0.0;
+    double S = 0.0;
+    ((S += cw[J] * density_k<kinds[J], (PROPS >> (4 * J)) & 15, cut_mode<PROPS, J, Ks...>()>(
+               Mk->comp[first + J], Mk->comp_flags[first + J], nd, fdir[J], cut)),
+     ...);
+    const double v = wk * S;
+    if (jt < n_last) {
+      lds_add(row + jt, v - v * f);
+      lds_add(row + jt + 1, v * f);
+    } else {
+      lds_add(row + n_last, v);
+    }
+  }
+}
+
+template <int SPLIT, uint32_t PROPS, int INTEGRAND, class... Gs, int... G>
+__device__ __forceinline__ void table_groups(Layout<INTEGRAND, Gs...>,
+                                             std::integer_sequence<int, G...>, const Ctx& cx,
+                                             const AdjArgs& a, const Ray& y, double gscale,
+                                             double* row, int sub) {
+  (table_group<INTEGRAND, SPLIT, props_shift(PROPS, group_first<Gs...>(G))>(
+       Gs{}, std::make_integer_sequence<int, Gs::size>{}, cx, a, y, G, group_first<Gs...>(G),
+       gscale, row, sub),
+   ...);
+}
+
+template <int SPLIT, uint32_t PROPS, int INTEGRAND, class... Gs>
+__device__ __forceinline__ void run_table(Layout<INTEGRAND, Gs...> lay, const Ctx& cx,
+                                          const AdjArgs& a, const Ray& y, double gscale,
+                                          double* row, int sub) {
+  table_groups<SPLIT, PROPS>(lay, std::make_integer_sequence<int, sizeof...(Gs)>{}, cx, a, y,
+                             gscale, row, sub);
+}
+
+// SPEC: the registry forms (RegistryProps, with the shared band cut-off when the
+// host finds the block allows it: SHARE).
+template <class LAYOUT, int SPLIT, bool SPEC, bool SHARE>
+__global__ __launch_bounds__(kAdjBlock) __attribute__((amdgpu_waves_per_eu(4))) void
+table_layout(AdjArgs a) {
+  constexpr uint32_t PROPS =
+      !SPEC ? 0u
+            : (SHARE ? RegistryProps<LAYOUT>::value
+                     : props_without_share(RegistryProps<LAYOUT>::value));
+  adjoint_body<SPLIT, false, 0>(
+      a, [&](const Ctx& cx, const Ray& y, double gscale, double* row, int, int sub, double*) {
+        run_table<SPLIT, PROPS>(LAYOUT{}, cx, a, y, gscale, row, sub);
+      });
+}
+
 template <int INTEGRAND, int SPLIT>
 __global__ __launch_bounds__(kAdjBlock) __attribute__((amdgpu_waves_per_eu(2))) void
 table_adjoint(AdjArgs a) {
@@ -123,11 +224,50 @@ AdjKernel pick_table_split(int split) {
   }
 }
 
+template <class LAYOUT, bool SPEC, bool SHARE>
+AdjKernel pick_table_layout_split(int split) {
+  switch (split) {
+    case 1:
+      return table_layout<LAYOUT, 1, SPEC, SHARE>;
+    case 2:
+      return table_layout<LAYOUT, 2, SPEC, SHARE>;
+    case 4:
+      return table_layout<LAYOUT, 4, SPEC, SHARE>;
+    default:
+      return nullptr;
+  }
+}
+
+template <class LAYOUT>
+AdjKernel pick_table_layout(const zodi_model* m, int split) {
+  constexpr uint32_t kProps = RegistryProps<LAYOUT>::value;
+  if constexpr (kProps != props_without_share(kProps)) {
+    if (props_match(m, kProps)) return pick_table_layout_split<LAYOUT, true, true>(split);
+  }
+  if (props_match(m, props_without_share(kProps)))
+    return pick_table_layout_split<LAYOUT, true, false>(split);
+  return pick_table_layout_split<LAYOUT, false, false>(split);
+}
+
 }  // namespace
 
-AdjKernel table_adjoint_kernel(int integrand, int split) {
-  if (integrand == ZODI_INTEGRAND_KELSALL) return pick_table_split<ZODI_INTEGRAND_KELSALL>(split);
-  if (integrand == ZODI_INTEGRAND_RRM) return pick_table_split<ZODI_INTEGRAND_RRM>(split);
+bool table_adjoint_is_layout(const zodi_model* m) { return match_layout(m) != kLayoutGeneric; }
+
+AdjKernel table_adjoint_kernel(const zodi_model* m, int split) {
+  switch (match_layout(m)) {
+    case kLayoutDirbe:
+      return pick_table_layout<LayoutDirbe>(m, split);
+    case kLayoutKelsall4:
+      return pick_table_layout<LayoutKelsall4>(m, split);
+    case kLayoutRRM:
+      return pick_table_layout<LayoutRRM>(m, split);
+    case kLayoutRRMSplit:
+      return pick_table_layout<LayoutRRMSplit>(m, split);
+    default:
+      break;
+  }
+  if (m->integrand == ZODI_INTEGRAND_KELSALL) return pick_table_split<ZODI_INTEGRAND_KELSALL>(split);
+  if (m->integrand == ZODI_INTEGRAND_RRM) return pick_table_split<ZODI_INTEGRAND_RRM>(split);
   return nullptr;
 }
 

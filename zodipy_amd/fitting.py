@@ -596,8 +596,13 @@ class BandpassFit:
         w, wn = self._normalised(weights)
         self.set_weights(w)
         blocks = (self._host, self._dev)
-        c2, g = self.problem.chi2_and_grad(None, data, inv_var, blocks=blocks)
-        _, g_table = self.problem.chi2_table_grad(None, data, inv_var, blocks=blocks)
+        prob = self.problem
+        c2, g, model = prob.chi2_and_grad(None, data, inv_var, return_model=True, blocks=blocks)
+        # the chi^2 weights 2 (I - d)/sigma^2 from this I: no second forward pass
+        iv_t, iv_s = prob._inv_var(inv_var)
+        gw = 2.0 * (model - prob._rows(data, (prob.n_rays,), "data"))
+        gw = gw * iv_t if iv_t is not None else gw * iv_s
+        g_table = prob.table_vjp(None, gw, blocks=blocks)
         d_wn = self._c * (self._bnu @ g_table)
         if len(self.names):
             d_wn = d_wn + self._K.t() @ g[self._eff_index]
