@@ -31,7 +31,9 @@ feature) and the RRM model (packed one component per group, since a parameter
 vector may give its components different temperature laws) -- run kernels
 specialised for them; any other combination of component kinds runs one
 generic kernel that takes a group's components one per pass over its nodes.
-``SpectralTableFit`` fits the spectral tables themselves on the device.
+``SpectralTableFit`` fits the spectral tables themselves on the device, and
+``BandpassFit`` the bandpass weights (with ``LineOfSightProblem.table_vjp``, the
+gradient with respect to the Planck table).
 """
 
 from __future__ import annotations
