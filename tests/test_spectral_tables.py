@@ -114,3 +114,52 @@ def test_planck_coefficients_reproduce_the_table(case):
     a = H * nu[:, None] / (KB * temps[None, :])
     b = (2 * H * nu[:, None] ** 3 / (C**2 * np.expm1(a))) / MJY
     np.testing.assert_allclose(q @ b, want, rtol=1e-13, atol=0)
+
+
+BANDPASS_CASES = [
+    ("dirbe", np.linspace(20.0, 30.0, 8), u.micron, False),
+    ("dirbe", np.linspace(10.0, 15.0, 6) * 1e3, u.GHz, False),
+    ("planck18", np.linspace(771.3, 942.7, 12), u.GHz, True),
+    ("rrm-experimental", np.linspace(20.0, 30.0, 8), u.micron, False),
+]
+
+
+@pytest.mark.parametrize("name,xs,unit,extrap", BANDPASS_CASES,
+                         ids=["dirbe_um", "dirbe_ghz_on_um", "planck18_857", "rrm_um"])
+def test_bandpass_maps_reproduce_the_unpacking(name, xs, unit, extrap):
+    """The linear maps behind bandpass-weight fits: at random weights, K @ wn equals
+    the unpacking of those weights (every tabulated parameter) and
+    sum_i c_i wn_i B_nu_i(T_k) the host's Planck table of them (1e-13)."""
+    from zodipy_amd.spectral import (
+        bandpass_maps,
+        effective_value,
+        normalize_weights,
+        planck_table,
+        unpack_spectral,
+    )
+    from zodipy_amd.zodiacal_light_model import model_registry
+
+    model = model_registry.get_model(name)
+    x = xs * unit
+    maps = bandpass_maps(model, x, extrap)
+    assert maps.K.shape == (len(maps.names), xs.size) and len(maps.names) > 0
+    rng = np.random.default_rng(17)
+    for _ in range(3):
+        w = rng.uniform(0.2, 1.5, xs.size)
+        wn = normalize_weights(x, w)
+        np.testing.assert_allclose(wn, w / (maps.c @ w), rtol=1e-15)
+        sp = unpack_spectral(model, x, wn, extrap)
+        want = np.array([effective_value(sp, n) for n in maps.names])
+        np.testing.assert_allclose(maps.K @ wn, want, rtol=1e-13, atol=1e-300)
+        np.testing.assert_allclose((maps.c * wn) @ maps.bnu, planck_table(x, wn)[1], rtol=1e-13)
+
+
+def test_trapezoid_coefficients():
+    from zodipy_amd.spectral import trapezoid_coefficients
+
+    rng = np.random.default_rng(5)
+    x = np.sort(rng.uniform(0, 10, 17))
+    y = rng.normal(size=17)
+    assert trapezoid_coefficients(x) @ y == pytest.approx(np.trapezoid(y, x), rel=1e-14)
+    assert trapezoid_coefficients(x[::-1]) @ y == pytest.approx(np.trapezoid(y, x[::-1]),
+                                                                 rel=1e-14)

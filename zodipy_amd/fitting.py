@@ -527,8 +527,7 @@ class BandpassFit:
     """
 
     def __init__(self, problem: LineOfSightProblem, theta=None) -> None:
-        from zodipy_amd import units as u
-        from zodipy_amd.spectral import planck_table, spectral_tables, unpack_spectral
+        from zodipy_amd.spectral import bandpass_maps
 
         self.problem = problem
         space = problem.space
@@ -536,34 +535,20 @@ class BandpassFit:
             msg = "the model has no bandpass (weights over two or more frequencies)"
             raise ValueError(msg)
         self.theta = space.theta0.copy() if theta is None else space._check(theta)
-        x = space._x
-        xv = np.asarray(x.value, dtype=np.float64)
-        self.n_freq = int(xv.size)
-        d = np.diff(xv)
-        c = np.zeros_like(xv)
-        c[:-1] += d / 2.0
-        c[1:] += d / 2.0
+        maps = bandpass_maps(space._ipd, space._x, space._extrapolate)
+        self.n_freq = int(maps.c.size)
         self.weights0 = np.asarray(space._normalized_weights, dtype=np.float64).copy()
-        tables = spectral_tables(space._ipd, x, self.weights0, space._extrapolate)
-        self.names = tuple(t.name for t in tables)
-        eye = np.eye(self.n_freq)
-        K = np.zeros((len(tables), self.n_freq))
-        for i in range(self.n_freq):
-            sp = unpack_spectral(space._ipd, x, eye[i], space._extrapolate)
-            for t, tab in enumerate(tables):
-                K[t, i] = _effective_value(sp, tab.name)
-        bnu = np.stack([planck_table(x[i:i + 1], None)[1] for i in range(self.n_freq)])
+        self.names = maps.names
         dev = problem.device
-        self._c = torch.as_tensor(c, device=dev)
-        self._K = torch.as_tensor(K, device=dev)
-        self._bnu = torch.as_tensor(bnu, device=dev)  # (n_freq, 100)
-        self._nu = torch.as_tensor(
-            np.asarray(x.to_value(u.Hz, equivalencies=u.spectral()), dtype=np.float64), device=dev)
-        self._targets = np.array([t.target for t in tables], dtype=np.int32)
-        self._eff_index = torch.as_tensor([space.index(t.name) for t in tables], device=dev,
+        self._c = torch.as_tensor(maps.c, device=dev)
+        self._K = torch.as_tensor(maps.K, device=dev)
+        self._bnu = torch.as_tensor(maps.bnu, device=dev)  # (n_freq, 100)
+        self._nu = torch.as_tensor(maps.nu, device=dev)
+        self._targets = maps.targets
+        self._eff_index = torch.as_tensor([space.index(n) for n in self.names], device=dev,
                                           dtype=torch.int64)
         scatter = tuple(lab for lab in space._ipd.comps
-                        if any(t.name == f"albedo.{lab.value}" for t in tables))
+                        if f"albedo.{lab.value}" in self.names)
         packed = space.pack(self.theta, force_scatter=scatter)
         self._host = np.ascontiguousarray(packed.block)
         self._dev = torch.from_numpy(self._host.view(np.uint8).copy()).to(dev)
@@ -611,18 +596,6 @@ class BandpassFit:
         S = torch.dot(self._c, w)
         d_w = (d_wn - self._c * torch.dot(wn, d_wn)) / S
         return c2, d_w
-
-
-def _effective_value(spectral: dict, name: str) -> float:
-    """The effective value named as in the gradient vector from an unpacked dict."""
-    if "." in name:
-        key, label = name.split(".", 1)
-        for lab, vals in spectral["comp"].items():
-            if lab.value == label:
-                return float(vals[key])
-        msg = f"no component {label!r}"
-        raise KeyError(msg)
-    return float(spectral["shared"][name])
 
 
 class _Emission(torch.autograd.Function):

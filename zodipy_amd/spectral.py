@@ -265,3 +265,69 @@ def planck_coefficients(x: u.Quantity, normalized_weights) -> tuple[np.ndarray, 
     c[:-1] += d / 2.0
     c[1:] += d / 2.0
     return nu, c * np.asarray(normalized_weights, dtype=np.float64)
+
+
+@dataclass(frozen=True)
+class BandpassMaps:
+    """The effective spectral parameters and the Planck table as linear maps of the
+    normalised bandpass weights wn at fixed frequencies x.
+
+    ``eff_t = K[t] @ wn`` for every tabulated parameter ``names[t]`` (set-up target
+    ``targets[t]``), K's column i being the unpacking of a unit weight at x_i
+    (zodipy/unpack_model.py:140-173: interpolation, extrapolation and unit rules
+    included); ``B(T_k) = sum_i c_i wn_i bnu[i, k]`` (zodipy/blackbody.py:33-49), c
+    the trapezoid coefficients over x, bnu the blackbody at each frequency and
+    table temperature [MJy/sr]; ``nu`` the frequencies in Hz. The model normalises
+    raw weights w as wn = w / (c . w) (zodipy/model.py:82-93).
+    """
+
+    names: tuple
+    targets: np.ndarray
+    K: np.ndarray
+    c: np.ndarray
+    bnu: np.ndarray
+    nu: np.ndarray
+
+
+def trapezoid_coefficients(xv: np.ndarray) -> np.ndarray:
+    """c with trapezoid(y, xv) = c @ y."""
+    xv = np.asarray(xv, dtype=np.float64)
+    d = np.diff(xv)
+    c = np.zeros_like(xv)
+    c[:-1] += d / 2.0
+    c[1:] += d / 2.0
+    return c
+
+
+def bandpass_maps(model: ZodiacalLightModel, x: u.Quantity, extrapolate: bool) -> BandpassMaps:
+    """BandpassMaps of ``model`` at the frequencies ``x`` (two or more)."""
+    xv = np.atleast_1d(np.asarray(x.value, dtype=np.float64))
+    n = xv.size
+    if n < 2:
+        msg = "a bandpass needs two or more frequencies"
+        raise ValueError(msg)
+    eye = np.eye(n)
+    tables = spectral_tables(model, x, np.full(n, 1.0 / np.ptp(xv)), extrapolate)
+    K = np.zeros((len(tables), n))
+    for i in range(n):
+        sp = unpack_spectral(model, x, eye[i], extrapolate)
+        for t, tab in enumerate(tables):
+            K[t, i] = effective_value(sp, tab.name)
+    bnu = np.stack([planck_table(x[i:i + 1], None)[1] for i in range(n)])
+    nu = np.asarray(x.to_value(u.Hz, equivalencies=u.spectral()), dtype=np.float64)
+    return BandpassMaps(tuple(t.name for t in tables),
+                        np.array([t.target for t in tables], dtype=np.int32), K,
+                        trapezoid_coefficients(xv), bnu, np.atleast_1d(nu))
+
+
+def effective_value(spectral: dict, name: str) -> float:
+    """The effective value named as in the gradient vector (``emissivity.cloud``,
+    ``C1``, ``calibration``, ...) from an ``unpack_spectral`` result."""
+    if "." in name:
+        key, label = name.split(".", 1)
+        for lab, vals in spectral["comp"].items():
+            if lab.value == label:
+                return float(vals[key])
+        msg = f"no component {label!r}"
+        raise KeyError(msg)
+    return float(spectral["shared"][name])
