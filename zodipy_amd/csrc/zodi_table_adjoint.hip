@@ -18,9 +18,10 @@
 //
 // One ray per lane (or SPLIT lanes sharing its nodes). The registry layouts run
 // with their kinds fixed at compile time (table_layout), any other layout through
-// the kind switch (table_adjoint). Each node adds its two shares
-// to the wave's row in LDS with wavefront-scope atomic adds (lanes of a wave may
-// meet in one entry); the block adds its wave rows in wave order and a reduction
+// the kind switch (table_adjoint). A lane sums the shares of consecutive nodes in
+// one table interval in registers and adds them to the wave's row in LDS with
+// wavefront-scope atomic adds when the interval changes (lanes of a wave may meet
+// in one entry); the block adds its wave rows in wave order and a reduction
 // kernel adds the blocks in block order.
 
 #include <hip/hip_runtime.h>
@@ -38,6 +39,37 @@ constexpr int kMaxGroupSlots = 4;  // components per group (packing.MAX_SLOTS_PE
 __device__ __forceinline__ void lds_add(double* p, double x) {
   __hip_atomic_fetch_add(p, x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
 }
+
+// A lane's pending shares of consecutive nodes in one table interval: entry j
+// gets s0, entry j + 1 gets s1 (none past the last entry). Nodes along a ray
+// change temperature slowly, so most of them only add to registers; the LDS
+// atomics run when the interval changes and at the end of the group.
+struct TablePending {
+  int j = -1;
+  double s0 = 0.0, s1 = 0.0;
+
+  __device__ __forceinline__ void flush(double* row, int n_last) {
+    if (j >= 0) {
+      lds_add(row + j, s0);
+      if (j < n_last) lds_add(row + j + 1, s1);
+    }
+  }
+  // One node at interval jt, fraction f, weight v.
+  __device__ __forceinline__ void add(double* row, int n_last, int jt, double f, double v) {
+    const int e = jt < n_last ? jt : n_last;
+    const double c0 = jt < n_last ? v - v * f : v;
+    const double c1 = jt < n_last ? v * f : 0.0;
+    if (e != j) {
+      flush(row, n_last);
+      j = e;
+      s0 = c0;
+      s1 = c1;
+    } else {
+      s0 += c0;
+      s1 += c1;
+    }
+  }
+};
 
 template <int INTEGRAND, int SPLIT>
 __device__ __forceinline__ void table_ray(const Ctx& cx, const AdjArgs& a, const Ray& y,
@@ -70,6 +102,7 @@ __device__ __forceinline__ void table_ray(const Ctx& cx, const AdjArgs& a, const
       }
     }
     const int K = y.active ? cx.K : 0;
+    TablePending pend;
     for (int k = sub; k < K; k += SPLIT) {
       const ModelPtr Mk = opaque(M);
       const double xk = Mk->gl_x[k];
@@ -92,14 +125,9 @@ __device__ __forceinline__ void table_ray(const Ctx& cx, const AdjArgs& a, const
         S += cw[q] * density(Mk->comp_kind[c], Mk->comp[c], Mk->comp_flags[c], X0, X1, X2, R,
                              inv_R, R2, fd[q]);
       }
-      const double v = wk * S;
-      if (jt < n_last) {
-        lds_add(row + jt, v - v * f);
-        lds_add(row + jt + 1, v * f);
-      } else {
-        lds_add(row + n_last, v);
-      }
+      pend.add(row, n_last, jt, f, wk * S);
     }
+    pend.flush(row, n_last);
   }
 }
 
@@ -135,6 +163,7 @@ __device__ __forceinline__ void table_group(Grp<Ks...>, std::integer_sequence<in
       }(),
       ...);
   const int K = y.active ? cx.K : 0;
+  TablePending pend;
   for (int k = sub; k < K; k += SPLIT) {
     const ModelPtr Mk = opaque(M);
     const double xk = Mk->gl_x[k];
@@ -157,14 +186,9 @@ __device__ __forceinline__ void table_group(Grp<Ks...>, std::integer_sequence<in
     ((S += cw[J] * density_k<kinds[J], (PROPS >> (4 * J)) & 15, cut_mode<PROPS, J, Ks...>()>(
                Mk->comp[first + J], Mk->comp_flags[first + J], nd, fdir[J], cut)),
      ...);
-    const double v = wk * S;
-    if (jt < n_last) {
-      lds_add(row + jt, v - v * f);
-      lds_add(row + jt + 1, v * f);
-    } else {
-      lds_add(row + n_last, v);
-    }
+    pend.add(row, n_last, jt, f, wk * S);
   }
+  pend.flush(row, n_last);
 }
 
 template <int SPLIT, uint32_t PROPS, int INTEGRAND, class... Gs, int... G>
