@@ -106,10 +106,10 @@ zodi_status validate_model(const zodi_model* m) {
 
 
 // Minimum waves per SIMD the register allocator must leave room for, per layout
-// (1 = no constraint). Measured on MI355X: DIRBE and Kelsall4 run at 5 (95 and
-// 82 VGPRs, no spill; DIRBE 2.7 % faster than at 4, and 6 waves with a spill
-// gains nothing); RRM at 4 (128 VGPRs with a small spill), 15 % faster than its
-// unconstrained 190 VGPRs / 2 waves.
+// (1 = no constraint). Measured on MI355X: DIRBE and Kelsall4 run at 5 (89 and
+// 80 VGPRs in their registry variants, no spill; DIRBE 2.7 % faster than at 4,
+// and 6 waves with a spill gains nothing); RRM at 4 (128 VGPRs with a small
+// spill), 15 % faster than its unconstrained 190 VGPRs / 2 waves.
 template <class LAYOUT>
 struct ForwardOccupancy {
   static constexpr int value = 1;

@@ -34,8 +34,6 @@ namespace zodi {
 
 namespace {
 
-constexpr int kMaxGroupSlots = 4;  // components per group (packing.MAX_SLOTS_PER_GROUP)
-
 __device__ __forceinline__ void lds_add(double* p, double x) {
   __hip_atomic_fetch_add(p, x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
 }
@@ -85,10 +83,10 @@ __device__ __forceinline__ void table_ray(const Ctx& cx, const AdjArgs& a, const
     const double us = M->group_T0[g] * cx.tb.inv_step;
     const double uo = cx.tb.T_first * cx.tb.inv_step;
     const double mdelta = -M->group_delta[g];
-    double cw[kMaxGroupSlots];
-    FeatureDir fd[kMaxGroupSlots];
+    double cw[kMaxSlots];
+    FeatureDir fd[kMaxSlots];
 #pragma unroll
-    for (int j = 0; j < kMaxGroupSlots; ++j) {
+    for (int j = 0; j < kMaxSlots; ++j) {
       cw[j] = 0.0;
       fd[j] = FeatureDir{1.0, 0.0};
       if (j < cnt) {
