@@ -537,6 +537,9 @@ class BandpassFit:
         self.theta = space.theta0.copy() if theta is None else space._check(theta)
         maps = bandpass_maps(space._ipd, space._x, space._extrapolate)
         self.n_freq = int(maps.c.size)
+        if self.n_freq > 256:  # zodi_spectral_setup's largest grid
+            msg = f"bandpass fits take up to 256 frequencies, the model has {self.n_freq}"
+            raise ValueError(msg)
         self.weights0 = np.asarray(space._normalized_weights, dtype=np.float64).copy()
         self.names = maps.names
         dev = problem.device
