@@ -453,3 +453,38 @@ def test_body_track_host_and_validation():
         BodyTrack(grid[:1], pos[:, :1])
     with pytest.raises(ValueError):
         BodyTrack(grid[::-1], pos)
+
+
+@pytest.mark.parametrize("n_obs", [2, 50000])
+def test_prepare_observers_uploads(n_obs):
+    """Observer data go to the device in one packed upload (small calls) or one
+    upload per array (large ones); either way the views hold the arrays' values
+    and dtypes. Checked with host memory standing in for the device."""
+    import torch
+
+    from zodipy_amd.runtime import DeviceModel
+
+    model = Model(25 * u.micron)
+    dm = DeviceModel.__new__(DeviceModel)
+    dm.packed = model._packed
+    dm.device = torch.device("cpu")
+    rng = np.random.default_rng(3)
+    obs = rng.uniform(-2, 2, (3, n_obs))
+    earth = obs * 1.01
+    seg = np.concatenate([[0], np.cumsum(rng.integers(0, 5, n_obs))])
+    s, n, o, e, b, mode = dm.prepare_observers(obs, earth, seg, False)
+    assert n == n_obs and mode == 0
+    assert s.dtype == torch.int64 and np.array_equal(s.numpy(), seg)
+    assert o.dtype == torch.float64 and np.array_equal(o.numpy(), obs)
+    assert np.array_equal(e.numpy(), earth)
+    want = model._packed.beyond_bits(obs, any_over_all=False).view(np.int32)
+    assert b.dtype == torch.int32 and np.array_equal(b.numpy(), want)
+    # per-ray observers from read-only broadcast views
+    ro = np.broadcast_to(obs[:, :1], (3, 7))
+    s, n, o, e, b, mode = dm.prepare_observers(ro, np.broadcast_to(earth[:, :1], (3, 7)), None,
+                                               True)
+    assert s is None and n == 0 and mode == 1
+    assert np.array_equal(o.numpy(), np.asarray(ro)) and tuple(e.shape) == (3, 7)
+    assert np.array_equal(b.numpy(),
+                          model._packed.beyond_bits(np.asarray(ro), any_over_all=True)
+                          .view(np.int32))

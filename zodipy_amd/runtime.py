@@ -50,6 +50,9 @@ def _f64_rows(t: torch.Tensor, rows: int, name: str, device) -> torch.Tensor:
     return t.contiguous()
 
 
+_TORCH_DTYPE = {np.float64: torch.float64, np.int64: torch.int64, np.int32: torch.int32}
+
+
 class DeviceModel:
     """A packed model resident on one GPU."""
 
@@ -121,12 +124,28 @@ class DeviceModel:
                 msg = "seg_offsets must start at 0 and be non-decreasing"
                 raise ValueError(msg)
             beyond = self.packed.beyond_bits(obs_np, any_over_all=False)
-            seg = torch.from_numpy(seg_np).to(self.device)
             mode = OBS_PER_SEGMENT
-        # np.array copies: broadcast views (per-ray observers) are read-only
-        obs = torch.from_numpy(np.array(obs_np, dtype=np.float64, order="C")).to(self.device)
-        earth = torch.from_numpy(np.array(earth_np, dtype=np.float64, order="C")).to(self.device)
-        beyond_t = torch.from_numpy(beyond.view(np.int32).copy()).to(self.device)
+        parts = [np.ascontiguousarray(obs_np, dtype=np.float64),
+                 np.ascontiguousarray(earth_np, dtype=np.float64),
+                 np.ascontiguousarray(beyond.view(np.int32))]
+        if mode == OBS_PER_SEGMENT:
+            parts.insert(0, np.ascontiguousarray(seg_np))
+        if sum(a.nbytes for a in parts) <= 1 << 20:
+            # One upload for all of them (a small call's fixed cost is mostly its
+            # copies): the arrays at 8-byte offsets of one buffer, viewed in place.
+            sizes = [a.nbytes for a in parts]
+            offs = np.concatenate([[0], np.cumsum([(n + 7) // 8 * 8 for n in sizes])]).astype(int)
+            buf = np.zeros(int(offs[-1]), dtype=np.uint8)
+            for a, o in zip(parts, offs):
+                buf[o:o + a.nbytes] = a.reshape(-1).view(np.uint8)
+            dev = torch.from_numpy(buf).to(self.device)
+            up = [dev[o:o + a.nbytes].view(_TORCH_DTYPE[a.dtype.type]).reshape(a.shape)
+                  for a, o in zip(parts, offs)]
+        else:
+            # np.array copies: broadcast views (per-ray observers) are read-only
+            up = [torch.from_numpy(np.array(a)).to(self.device) for a in parts]
+        seg = up.pop(0) if mode == OBS_PER_SEGMENT else None
+        obs, earth, beyond_t = up
         return seg, n_seg, obs, earth, beyond_t, mode
 
     def forward_raw(self, u: torch.Tensor, seg, n_seg: int, obs: torch.Tensor,
