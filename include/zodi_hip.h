@@ -30,6 +30,9 @@
  *   zodi_table_vjp        no reference counterpart: the gradient with respect to
  *                         the Planck table (zodipy/blackbody.py:33-49), for fits of
  *                         the bandpass weights
+ *   zodi_spectral_vjp     no reference counterpart: the same together with the
+ *                         gradient with respect to the effective spectral
+ *                         parameters, for bandpass and spectral-table fits
  *   zodi_spectral_setup   zodipy/unpack_model.py:23-173 and zodipy/blackbody.py:33-49
  *                         (effective spectral parameters and the Planck table,
  *                         written into a device model block)
@@ -355,8 +358,10 @@ zodi_status zodi_adjoint(const zodi_model* model_host, const zodi_model* model_d
  * contributes (1 - f) and f; a node at or past the last entry, or clamped below
  * the first, all of it to that entry. workspace: zodi_table_vjp_workspace bytes
  * (block partials, then I unless model_out is given). launch: block,
- * lanes_per_ray and blocks_per_cu as in zodi_adjoint; variant is ignored. Any component layout. The reduction order is fixed by the launch
- * shape; within a wave, lanes that meet in one entry add in hardware order.
+ * lanes_per_ray and blocks_per_cu as in zodi_adjoint (blocks_per_cu 0: as many
+ * as the kernel keeps resident); variant is ignored. Any component layout. The
+ * reduction order is fixed by the launch shape; within a wave, lanes that meet
+ * in one entry add in hardware order.
  */
 int64_t zodi_table_vjp_workspace(const zodi_model* model_host, int64_t n_rays,
                                  const zodi_launch* launch);
@@ -369,6 +374,33 @@ zodi_status zodi_table_vjp(const zodi_model* model_host, const zodi_model* model
                            double inv_var_scalar, double* model_out, double* grad_table,
                            double* workspace, int64_t workspace_bytes, const zodi_launch* launch,
                            void* stream);
+
+/*
+ * zodi_spectral_vjp — zodi_table_vjp together with the gradient with respect to
+ * the effective spectral parameters of zodipy/brightness.py:21-83 (emissivity
+ * and albedo of each Kelsall component, the solar irradiance and C1..C3; the
+ * RRM calibration), in which the integrand is linear: everything a fit of the
+ * bandpass weights or of the spectral tables (zodipy/unpack_model.py:140-173)
+ * needs, in one pass over the nodes at about the cost of a forward pass, in
+ * place of zodi_adjoint's full gradient. No reference counterpart (the reference
+ * has no derivatives). grad [n_params + ZODI_N_TEMPS + 1]: entries 0..n_params-1
+ * in zodi_adjoint's order, the effective spectral parameters' filled (they agree
+ * with zodi_adjoint's to rounding) and every other entry 0; then the table
+ * gradient as zodi_table_vjp writes it; then chi^2 (ZODI_ADJ_CHI2; else 0).
+ * Other arguments, workspace (zodi_spectral_vjp_workspace bytes) and launch as
+ * in zodi_table_vjp; the block's gradient layout must be valid (zodi_adjoint).
+ */
+int64_t zodi_spectral_vjp_workspace(const zodi_model* model_host, int64_t n_rays,
+                                    const zodi_launch* launch);
+
+zodi_status zodi_spectral_vjp(const zodi_model* model_host, const zodi_model* model_dev,
+                              const double* u, int64_t n_rays, const int64_t* seg_offsets,
+                              int32_t n_seg, const double* obs, const double* earth,
+                              int32_t obs_mode, const uint32_t* beyond, int32_t mode,
+                              const double* weights, const double* target, const double* inv_var,
+                              double inv_var_scalar, double* model_out, double* grad,
+                              double* workspace, int64_t workspace_bytes,
+                              const zodi_launch* launch, void* stream);
 
 /*
  * zodi_adjoint_noalloc — zodi_adjoint without a workspace: no reference

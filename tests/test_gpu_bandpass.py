@@ -1,10 +1,14 @@
-"""GPU tests of the Planck-table gradient (zodi_table_vjp) and of bandpass-weight fits.
+"""GPU tests of the Planck-table gradient (zodi_table_vjp), of the spectral
+gradient (zodi_spectral_vjp) and of bandpass-weight fits.
 
 The table gradient is checked against central differences of chi^2 with one
 table entry B_k = B(T_k) of the device block moved at a time, and against the
-weighted (VJP) modes; the bandpass fit's forward against the model built from
-the same weights by the host packer, and its gradient against central
-differences over each weight. HEALPix nside 8 pixel centres, observer (1, 0, 0).
+weighted (VJP) modes; the spectral gradient's entries against zodi_adjoint's
+for the same parameters, on every kernel variant (registry layouts with and
+without the scattering sums, the kind switch); the bandpass fit's forward
+against the model built from the same weights by the host packer, and its
+gradient against central differences over each weight. HEALPix nside 8 pixel
+centres, observer (1, 0, 0).
 """
 
 from __future__ import annotations
@@ -148,3 +152,118 @@ def test_bandpass_fit_gradient_matches_central_fd(gpu, name, xs, unit, extrap):
     wt = torch.tensor(w, dtype=torch.float64, device="cuda", requires_grad=True)
     chi2_bandpass(wt, fit, data, iv).backward()
     np.testing.assert_allclose(wt.grad.cpu().numpy(), g, rtol=1e-12, atol=0)
+
+
+SPECTRAL_PREFIXES = ("emissivity.", "albedo.")
+SPECTRAL_SHARED = ("solar_irradiance", "C1", "C2", "C3", "calibration")
+
+
+def _spectral_mask(prob):
+    return np.array([n.startswith(SPECTRAL_PREFIXES) or n in SPECTRAL_SHARED
+                     for n in prob.space.names])
+
+
+def _generic_rrm():
+    """A registered RRM model of the fan and the comet only: no RRM registry layout."""
+    from zodipy_amd import component_params as cp
+    from zodipy_amd import source_params as sp
+    from zodipy_amd.component import ComponentLabel as L
+    from zodipy_amd.zodiacal_light_model import RRM, model_registry
+
+    name = "test-rrm-fan-comet"
+    try:
+        model_registry.get_model(name)
+    except ValueError:
+        keep = [L.FAN, L.COMET]
+        model_registry.register_model(name, RRM(
+            comps={k: cp.RRM[k] for k in keep}, spectrum=sp.SPECTRUM_IRAS,
+            T_0={k: sp.T_0_RRM[k] for k in keep}, delta={k: sp.DELTA_RMM[k] for k in keep},
+            calibration=sp.CALIBRATION_RRM))
+    return name
+
+
+SPECTRAL_CASES = ["dirbe25", "dirbe1.25_scatter", "planck18_style_857", "planck18_857", "rrm25",
+                  "cloud_ring1.25_generic", "rrm_fan_comet_generic"]
+
+
+def _spectral_model(case, planck18_style, cloud_ring_model):
+    if case == "dirbe25":  # DIRBE layout; F != 0, so the scattering sums run
+        return Model(25 * u.micron)
+    if case == "dirbe1.25_scatter":  # scattering components, angle-dependent phase
+        return Model(1.25 * u.micron)
+    if case == "planck18_style_857":  # DIRBE layout without scattering
+        return Model(857 * u.GHz, name=planck18_style)
+    if case == "planck18_857":  # four-component layout without scattering
+        return Model(857 * u.GHz, name="planck18")
+    if case == "rrm25":
+        return Model(25 * u.micron, name="rrm-experimental")
+    if case == "cloud_ring1.25_generic":
+        return Model(1.25 * u.micron, name=cloud_ring_model)
+    return Model(25 * u.micron, name=_generic_rrm())
+
+
+def _assert_spectral_entries(got, want, mask, what):
+    got = got.cpu().numpy()
+    want = want.cpu().numpy()
+    assert np.all(got[~mask] == 0.0), what
+    scale = float(np.abs(want[mask]).max())
+    assert scale > 0.0, what
+    np.testing.assert_allclose(got[mask], want[mask], rtol=1e-10, atol=1e-12 * scale,
+                               err_msg=what)
+
+
+@pytest.mark.parametrize("case", SPECTRAL_CASES)
+def test_spectral_vjp_matches_adjoint(gpu, case, planck18_style, cloud_ring_model):
+    """zodi_spectral_vjp's parameter entries equal zodi_adjoint's for the effective
+    spectral parameters (the same partials, summed in another order) and are 0
+    elsewhere; its table part equals zodi_table_vjp's; in the weighted modes and
+    in chi^2 mode (against the Jacobian path), with the model values it writes."""
+    prob = _problem(_spectral_model(case, planck18_style, cloud_ring_model))
+    theta = prob.space.theta0
+    mask = _spectral_mask(prob)
+    rng = np.random.default_rng(11)
+    w = torch.as_tensor(rng.normal(size=prob.n_rays), device="cuda")
+    g_th, g_tab = prob.spectral_vjp(theta, w)
+    _assert_spectral_entries(g_th, prob.vjp(theta, w), mask, f"{case}: vjp")
+    gt = prob.table_vjp(theta, w)
+    torch.testing.assert_close(g_tab, gt, rtol=1e-12, atol=1e-12 * float(gt.abs().max()))
+
+    wc = torch.as_tensor(rng.normal(size=(prob.n_comps, prob.n_rays)), device="cuda")
+    g_th, g_tab = prob.spectral_vjp(theta, wc, return_comps=True)
+    _assert_spectral_entries(g_th, prob.vjp(theta, wc, return_comps=True), mask,
+                             f"{case}: vjp per component")
+    gt = prob.table_vjp(theta, wc, return_comps=True)
+    torch.testing.assert_close(g_tab, gt, rtol=1e-12, atol=1e-12 * float(gt.abs().max()))
+
+    data, iv = _data(prob, theta, 12)
+    c2, g_th, g_tab, model = prob.chi2_spectral_grad(theta, data, iv, return_model=True)
+    c2_j, g_j = prob.chi2_and_grad(theta, data, iv)
+    assert float(c2) == pytest.approx(float(c2_j), rel=1e-12)
+    _assert_spectral_entries(g_th, g_j, mask, f"{case}: chi2")
+    c2_t, gt = prob.chi2_table_grad(theta, data, iv)
+    assert float(c2) == float(c2_t)
+    torch.testing.assert_close(g_tab, gt, rtol=1e-12, atol=1e-12 * float(gt.abs().max()))
+    torch.testing.assert_close(model, prob.forward(theta), rtol=1e-14, atol=0)
+
+
+def test_spectral_vjp_empty_and_launches(gpu):
+    """No rays: zeros; forced lane splits agree with the default launch."""
+    from zodipy_amd._lib import Launch
+
+    model = Model(25 * u.micron)
+    prob = _problem(model)
+    theta = prob.space.theta0
+    w = torch.as_tensor(np.random.default_rng(13).normal(size=prob.n_rays), device="cuda")
+    g0, t0 = prob.spectral_vjp(theta, w)
+    for split in (2, 4):
+        prob.launch = Launch(0, split, 0, 0)
+        try:
+            g, t = prob.spectral_vjp(theta, w)
+        finally:
+            prob.launch = None
+        torch.testing.assert_close(g, g0, rtol=1e-11, atol=1e-13 * float(g0.abs().max()))
+        torch.testing.assert_close(t, t0, rtol=1e-11, atol=1e-13 * float(t0.abs().max()))
+    empty = LineOfSightProblem(model, np.zeros((3, 0)), OBS, OBS)
+    none = torch.zeros(0, dtype=torch.float64, device="cuda")
+    c2, g, t = empty.chi2_spectral_grad(theta, none)
+    assert float(c2) == 0.0 and torch.all(g == 0) and torch.all(t == 0)

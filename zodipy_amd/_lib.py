@@ -37,6 +37,8 @@ EXPORTED_SYMBOLS = (
     "zodi_adjoint_noalloc",
     "zodi_table_vjp_workspace",
     "zodi_table_vjp",
+    "zodi_spectral_vjp_workspace",
+    "zodi_spectral_vjp",
     "zodi_jacobian",
     "zodi_spectral_setup",
 )
@@ -97,6 +99,12 @@ def _bind(lib: ct.CDLL) -> None:
     lib.zodi_table_vjp.restype = ct.c_int32
     lib.zodi_table_vjp.argtypes = [_p, _p, _p, _i64, _p, _i32, _p, _p, _i32, _p, _i32, _p, _p,
                                    _p, ct.c_double, _p, _p, _p, _i64, ct.POINTER(Launch), _p]
+    lib.zodi_spectral_vjp_workspace.restype = _i64
+    lib.zodi_spectral_vjp_workspace.argtypes = [_p, _i64, ct.POINTER(Launch)]
+    lib.zodi_spectral_vjp.restype = ct.c_int32
+    lib.zodi_spectral_vjp.argtypes = [_p, _p, _p, _i64, _p, _i32, _p, _p, _i32, _p, _i32, _p,
+                                      _p, _p, ct.c_double, _p, _p, _p, _i64, ct.POINTER(Launch),
+                                      _p]
     lib.zodi_spectral_setup.restype = ct.c_int32
     lib.zodi_spectral_setup.argtypes = [_p, _p, _i32, _i32, _p, _p, _p, _i32, _p, _p, _p]
     lib.zodi_jacobian.restype = ct.c_int32

@@ -263,13 +263,6 @@ AdjKernel pick_adj_chunk(int chunk, int split, bool jac, bool spec, bool share, 
   }
 }
 
-// A Kelsall block whose components all skip scattering and whose solar
-// irradiance is 0: every scattering sum of the adjoint is an exact zero.
-bool block_noscat(const zodi_model* m) {
-  return m->integrand == ZODI_INTEGRAND_KELSALL && (m->flags & 1) == 0 &&
-         m->shared[ZODI_S_SOLAR] == 0.0;
-}
-
 template <class LAYOUT>
 AdjKernel pick_adj_layout(const zodi_model* m, int chunk, int split, bool jac) {
   constexpr uint32_t kProps = RegistryProps<LAYOUT>::value;
@@ -408,19 +401,32 @@ zodi_status adjoint_plan(const zodi_launch* launch, int64_t n_rays, bool jac, Ad
   return ZODI_OK;
 }
 
-// Rows of the Planck-table gradient: one entry per table temperature, then chi^2.
+// Rows of the Planck-table gradient: one entry per table temperature, then chi^2;
+// with the effective spectral parameters (eff), the parameter vector first.
 constexpr int kTableRow = ZODI_N_TEMPS + 1;
 
-// Launch plan of the table adjoint: the weighted pass's, with as many resident
-// blocks per CU as its kernel is compiled for (four for the registry layouts, two
-// for the kind switch) unless the caller sets blocks_per_cu; the
-// components-per-pass variant does not apply.
-zodi_status table_plan(const zodi_model* m, const zodi_launch* launch, int64_t n_rays,
-                       AdjPlan& p) {
+int table_row(const zodi_model* m, bool eff) { return kTableRow + (eff ? m->n_params : 0); }
+
+// Launch plan and kernel of the table gradient: the weighted pass's plan, with as
+// many resident blocks per CU as the kernel is compiled for unless the caller
+// sets blocks_per_cu; the components-per-pass variant does not apply.
+zodi_status table_plan(const zodi_model* m, const zodi_launch* launch, int64_t n_rays, bool eff,
+                       AdjPlan& p, AdjKernel& k) {
+  if (eff) {
+    const zodi_status st = validate_grad_layout(m);
+    if (st != ZODI_OK) return st;
+  }
+  const TableKernel any = table_adjoint_kernel(m, 1, eff);
+  if (any.k == nullptr) return fail(ZODI_ERR_MODEL, "no table adjoint kernel for this model");
   zodi_launch l = launch != nullptr ? *launch : zodi_launch{0, 0, 0, 0};
-  if (l.blocks_per_cu == 0) l.blocks_per_cu = table_adjoint_is_layout(m) ? 4 : 2;
+  if (l.blocks_per_cu == 0) l.blocks_per_cu = any.waves;
   l.variant = 0;
-  return adjoint_plan(&l, n_rays, false, p);
+  const zodi_status st = adjoint_plan(&l, n_rays, false, p);
+  if (st != ZODI_OK) return st;
+  k = table_adjoint_kernel(m, p.split, eff).k;
+  if (k == nullptr)
+    return fail(ZODI_ERR_ARG, "no table adjoint kernel for %d lanes per ray", p.split);
+  return ZODI_OK;
 }
 
 // Largest Jacobian the chi^2 mode stores in its workspace (a quarter of one
@@ -553,6 +559,50 @@ zodi_status adjoint_recompute(const zodi_model* model_host, const zodi_model* mo
   hipLaunchKernelGGL(adjoint_reduce, dim3(np1), dim3(256), 0, (hipStream_t)stream,
                      (const double*)partials, (int)plan.grid, np1, grad);
   return check_hip(hipGetLastError(), "adjoint reduce launch");
+}
+
+int64_t table_workspace(const zodi_model* m, int64_t n_rays, const zodi_launch* launch,
+                        bool eff) {
+  if (m == nullptr || validate_model(m) != ZODI_OK || n_rays < 0) return -1;
+  AdjPlan p;
+  AdjKernel k = nullptr;
+  if (table_plan(m, launch, n_rays, eff, p, k) != ZODI_OK) return -1;
+  return recompute_bytes(p, table_row(m, eff), n_rays);
+}
+
+// zodi_table_vjp (eff false) and zodi_spectral_vjp (eff true): the recompute path
+// with the table kernel, whose rows take the place of the parameter rows.
+zodi_status table_vjp(bool eff, const zodi_model* model_host, const zodi_model* model_dev,
+                      const RayArgs& r, int32_t mode, const double* weights,
+                      const double* target, const double* inv_var, double inv_var_scalar,
+                      double* model_out, double* grad, double* workspace,
+                      int64_t workspace_bytes, const zodi_launch* launch, void* stream) {
+  zodi_status st = check_rays(model_host, model_dev, r.u, r.n_rays, r.seg_offsets, r.n_seg,
+                              r.obs, r.earth, r.obs_mode, r.beyond);
+  if (st != ZODI_OK) return st;
+  if (grad == nullptr || workspace == nullptr)
+    return fail(ZODI_ERR_ARG, "null pointer (grad or workspace)");
+  if (mode == ZODI_ADJ_CHI2) {
+    if (r.n_rays > 0 && target == nullptr) return fail(ZODI_ERR_ARG, "chi2 mode needs target");
+  } else if (mode == ZODI_ADJ_VJP_SUM || mode == ZODI_ADJ_VJP_COMPS) {
+    if (r.n_rays > 0 && weights == nullptr) return fail(ZODI_ERR_ARG, "vjp mode needs weights");
+  } else {
+    return fail(ZODI_ERR_ARG, "unknown adjoint mode %d", mode);
+  }
+  AdjPlan plan;
+  AdjKernel k = nullptr;
+  st = table_plan(model_host, launch, r.n_rays, eff, plan, k);
+  if (st != ZODI_OK) return st;
+  const int row = table_row(model_host, eff);
+  const int64_t need = recompute_bytes(plan, row, r.n_rays);
+  if (workspace_bytes < need)
+    return fail(ZODI_ERR_ARG, "workspace of %lld bytes needed, %lld given", (long long)need,
+                (long long)workspace_bytes);
+  double* vals = model_out != nullptr ? model_out : workspace + plan.grid * row;
+  zodi_model shape = *model_host;
+  shape.n_params = row - 1;
+  return adjoint_recompute(&shape, model_dev, r, mode, weights, target, inv_var, inv_var_scalar,
+                           plan, k, workspace, vals, grad, stream);
 }
 
 // Device address of this GPU's copy of `scratch`, looked up once per device with
@@ -704,10 +754,12 @@ zodi_status zodi_adjoint(const zodi_model* model_host, const zodi_model* model_d
 
 int64_t zodi_table_vjp_workspace(const zodi_model* model_host, int64_t n_rays,
                                  const zodi_launch* launch) {
-  if (model_host == nullptr || validate_model(model_host) != ZODI_OK || n_rays < 0) return -1;
-  AdjPlan p;
-  if (table_plan(model_host, launch, n_rays, p) != ZODI_OK) return -1;
-  return recompute_bytes(p, kTableRow, n_rays);
+  return table_workspace(model_host, n_rays, launch, false);
+}
+
+int64_t zodi_spectral_vjp_workspace(const zodi_model* model_host, int64_t n_rays,
+                                    const zodi_launch* launch) {
+  return table_workspace(model_host, n_rays, launch, true);
 }
 
 zodi_status zodi_table_vjp(const zodi_model* model_host, const zodi_model* model_dev,
@@ -718,35 +770,23 @@ zodi_status zodi_table_vjp(const zodi_model* model_host, const zodi_model* model
                            double inv_var_scalar, double* model_out, double* grad_table,
                            double* workspace, int64_t workspace_bytes, const zodi_launch* launch,
                            void* stream) {
-  zodi_status st = check_rays(model_host, model_dev, u, n_rays, seg_offsets, n_seg, obs, earth,
-                              obs_mode, beyond);
-  if (st != ZODI_OK) return st;
-  if (grad_table == nullptr || workspace == nullptr)
-    return fail(ZODI_ERR_ARG, "null pointer (grad_table or workspace)");
-  if (mode == ZODI_ADJ_CHI2) {
-    if (n_rays > 0 && target == nullptr) return fail(ZODI_ERR_ARG, "chi2 mode needs target");
-  } else if (mode == ZODI_ADJ_VJP_SUM || mode == ZODI_ADJ_VJP_COMPS) {
-    if (n_rays > 0 && weights == nullptr) return fail(ZODI_ERR_ARG, "vjp mode needs weights");
-  } else {
-    return fail(ZODI_ERR_ARG, "unknown adjoint mode %d", mode);
-  }
-  AdjPlan plan;
-  st = table_plan(model_host, launch, n_rays, plan);
-  if (st != ZODI_OK) return st;
-  AdjKernel k = table_adjoint_kernel(model_host, plan.split);
-  if (k == nullptr)
-    return fail(ZODI_ERR_ARG, "no table adjoint kernel for %d lanes per ray", plan.split);
-  const int64_t need = recompute_bytes(plan, kTableRow, n_rays);
-  if (workspace_bytes < need)
-    return fail(ZODI_ERR_ARG, "workspace of %lld bytes needed, %lld given", (long long)need,
-                (long long)workspace_bytes);
-  double* vals = model_out != nullptr ? model_out : workspace + plan.grid * kTableRow;
   const RayArgs r = {u, n_rays, seg_offsets, n_seg, obs, earth, obs_mode, beyond};
-  // the table's entries take the place of the parameters: rows of ZODI_N_TEMPS + 1
-  zodi_model shape = *model_host;
-  shape.n_params = ZODI_N_TEMPS;
-  return adjoint_recompute(&shape, model_dev, r, mode, weights, target, inv_var, inv_var_scalar,
-                           plan, k, workspace, vals, grad_table, stream);
+  return table_vjp(false, model_host, model_dev, r, mode, weights, target, inv_var,
+                   inv_var_scalar, model_out, grad_table, workspace, workspace_bytes, launch,
+                   stream);
+}
+
+zodi_status zodi_spectral_vjp(const zodi_model* model_host, const zodi_model* model_dev,
+                              const double* u, int64_t n_rays, const int64_t* seg_offsets,
+                              int32_t n_seg, const double* obs, const double* earth,
+                              int32_t obs_mode, const uint32_t* beyond, int32_t mode,
+                              const double* weights, const double* target, const double* inv_var,
+                              double inv_var_scalar, double* model_out, double* grad,
+                              double* workspace, int64_t workspace_bytes,
+                              const zodi_launch* launch, void* stream) {
+  const RayArgs r = {u, n_rays, seg_offsets, n_seg, obs, earth, obs_mode, beyond};
+  return table_vjp(true, model_host, model_dev, r, mode, weights, target, inv_var,
+                   inv_var_scalar, model_out, grad, workspace, workspace_bytes, launch, stream);
 }
 
 zodi_status zodi_adjoint_noalloc(const zodi_model* model_host, const zodi_model* model_dev,

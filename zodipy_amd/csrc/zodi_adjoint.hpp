@@ -535,11 +535,30 @@ __device__ __forceinline__ void adjoint_body(const AdjArgs& a, RAY ray) {
   }
 }
 
+// A Kelsall block whose components all skip scattering and whose solar
+// irradiance is 0: every scattering sum of the adjoint is an exact zero.
+inline bool block_noscat(const zodi_model* m) {
+  return m->integrand == ZODI_INTEGRAND_KELSALL && (m->flags & 1) == 0 &&
+         m->shared[ZODI_S_SOLAR] == 0.0;
+}
+
 // Kernel entry points of the other adjoint source files: custom component
-// layouts, and the gradient with respect to the Planck table.
+// layouts, and the gradient with respect to the Planck table (with eff, also
+// the effective spectral parameters).
 using AdjKernel = void (*)(AdjArgs);
 AdjKernel generic_adjoint_kernel(int integrand, int split, bool jac);
-AdjKernel table_adjoint_kernel(const zodi_model* m, int split);
-bool table_adjoint_is_layout(const zodi_model* m);  // the kernel is a registry layout's
+
+// Resident waves per SIMD the table kernels are compiled for (their register
+// budgets): the Kelsall registry layouts; the RRM ones, and the Kelsall ones with
+// the spectral sums of scattering components; the kind switch.
+constexpr int kTableWavesLayout = 4;
+constexpr int kTableWavesWide = 3;
+constexpr int kTableWavesGeneric = 2;
+
+struct TableKernel {
+  AdjKernel k;
+  int waves;  // resident blocks per CU of kAdjBlock threads
+};
+TableKernel table_adjoint_kernel(const zodi_model* m, int split, bool eff);
 
 }  // namespace zodi

@@ -343,25 +343,29 @@ class LineOfSightProblem:
         return t.contiguous()
 
     def _table_adjoint(self, theta, mode: int, weights=None, target=None, inv_var=None,
-                       inv_var_scalar: float = 1.0, blocks=None) -> torch.Tensor:
+                       inv_var_scalar: float = 1.0, model_out=None, blocks=None,
+                       eff: bool = False) -> torch.Tensor:
+        """zodi_table_vjp (eff False) or zodi_spectral_vjp (eff True): the gradient row."""
         host, dev = blocks if blocks is not None else self._model(theta)[:2]
-        need = int(self.lib.zodi_table_vjp_workspace(_lib.host_ptr(host), self.n_rays,
-                                                     self.launch))
+        name = "zodi_spectral_vjp" if eff else "zodi_table_vjp"
+        need = int(getattr(self.lib, name + "_workspace")(_lib.host_ptr(host), self.n_rays,
+                                                          self.launch))
         if need < 0:
             err = self.lib.zodi_last_error().decode(errors="replace")
-            msg = f"zodi_table_vjp_workspace rejected the model or launch: {err}"
+            msg = f"{name}_workspace rejected the model or launch: {err}"
             raise RuntimeError(msg)
         ws = torch.empty(max((need + 7) // 8, 1), dtype=torch.float64, device=self.device)
-        grad = torch.empty(N_TEMPS + 1, dtype=torch.float64, device=self.device)
+        n_row = N_TEMPS + 1 + (self.n_params if eff else 0)
+        grad = torch.empty(n_row, dtype=torch.float64, device=self.device)
 
         def ptr(t):
             return t.data_ptr() if t is not None else None
 
-        st = self.lib.zodi_table_vjp(
+        st = getattr(self.lib, name)(
             _lib.host_ptr(host), dev.data_ptr(), *self._ptr_args(host), mode, ptr(weights),
-            ptr(target), ptr(inv_var), float(inv_var_scalar), None, grad.data_ptr(),
+            ptr(target), ptr(inv_var), float(inv_var_scalar), ptr(model_out), grad.data_ptr(),
             ws.data_ptr(), need, self.launch, _stream())
-        _lib.check(st, "zodi_table_vjp")
+        _lib.check(st, name)
         return grad
 
     def table_vjp(self, theta, weights, return_comps: bool = False, blocks=None) -> torch.Tensor:
@@ -383,6 +387,37 @@ class LineOfSightProblem:
         g = self._table_adjoint(theta, ADJ_CHI2, target=d, inv_var=iv_t, inv_var_scalar=iv_s,
                                 blocks=blocks)
         return g[-1], g[:-1]
+
+    def spectral_vjp(self, theta, weights, return_comps: bool = False, blocks=None):
+        """(g_theta (n_params,), g_table (100,)): ``table_vjp`` together with the
+        entries of ``vjp`` for the effective spectral parameters (emissivities,
+        albedos, solar irradiance, C1..C3; RRM calibration), every other entry of
+        g_theta 0, from one pass over the nodes (``zodi_spectral_vjp``)."""
+        if return_comps:
+            w = self._rows(weights, (self.n_comps, self.n_rays), "weights")
+            g = self._table_adjoint(theta, ADJ_VJP_COMPS, weights=w, blocks=blocks, eff=True)
+        else:
+            w = self._rows(weights, (self.n_rays,), "weights")
+            g = self._table_adjoint(theta, ADJ_VJP_SUM, weights=w, blocks=blocks, eff=True)
+        n = self.n_params
+        return g[:n], g[n:-1]
+
+    def chi2_spectral_grad(self, theta, data, inv_var=None, return_model: bool = False,
+                           blocks=None):
+        """(chi^2, g_theta (n_params,), g_table (100,)[, model (n,)]): chi^2 and its
+        gradient with respect to the effective spectral parameters (other entries
+        of g_theta 0) and the Planck table entries, in one library call (the
+        forward pass, then one pass of ``zodi_spectral_vjp``)."""
+        d = self._rows(data, (self.n_rays,), "data")
+        iv_t, iv_s = self._inv_var(inv_var)
+        model = (torch.empty(self.n_rays, dtype=torch.float64, device=self.device)
+                 if return_model else None)
+        g = self._table_adjoint(theta, ADJ_CHI2, target=d, inv_var=iv_t, inv_var_scalar=iv_s,
+                                model_out=model, blocks=blocks, eff=True)
+        n = self.n_params
+        if return_model:
+            return g[-1], g[:n], g[n:-1], model
+        return g[-1], g[:n], g[n:-1]
 
     def _inv_var(self, inv_var):
         """(per-ray tensor or None, scalar) from None, a scalar or an (n,) array."""
@@ -433,7 +468,9 @@ class SpectralTableFit:
     ``zodi_spectral_setup`` writes A @ table and its derived slots into the fit's
     own device block, the chi^2 mode of ``zodi_adjoint`` gives d chi^2/d(effective
     values), and the chain rule d chi^2/d table[t, j] = A[t, j] d chi^2/d eff_t
-    runs on the device. Components with albedo tables are packed with their
+    runs on the device. (``chi2_spectral_grad`` gives the same entries from a
+    forward pass and the spectral sums alone; with the Jacobian pass's registry
+    kernels both cost about the same, so the fit keeps the full gradient.) Components with albedo tables are packed with their
     scattering branch, so an albedo may leave 0.
     """
 
@@ -515,9 +552,9 @@ class BandpassFit:
     column i the unpacking of a unit weight at x_i (every interpolation,
     extrapolation and unit rule of the unpacking itself), and
     B_k = sum_i c_i wn_i B_nu_i(T_k). Per call ``zodi_spectral_setup`` writes both
-    into the fit's own device block; chi^2 and d chi^2/d eff come from
-    ``zodi_adjoint``, d chi^2/dB_k from ``zodi_table_vjp``, and the chain rule,
-    through the normalisation, runs on the device:
+    into the fit's own device block; chi^2, d chi^2/d eff and d chi^2/dB_k come
+    from one ``zodi_spectral_vjp`` call (a forward pass, then one pass over the
+    nodes), and the chain rule, through the normalisation, runs on the device:
 
         d chi^2/d wn_i = sum_t K[t, i] d chi^2/d eff_t + c_i sum_k B_nu_i(T_k) d chi^2/dB_k,
         d chi^2/d w_j = (d chi^2/d wn_j - c_j sum_i wn_i d chi^2/d wn_i) / (c . w).
@@ -585,14 +622,8 @@ class BandpassFit:
         """(chi^2, d chi^2 / d weights (n_freq,)) on the device."""
         w, wn = self._normalised(weights)
         self.set_weights(w)
-        blocks = (self._host, self._dev)
-        prob = self.problem
-        c2, g, model = prob.chi2_and_grad(None, data, inv_var, return_model=True, blocks=blocks)
-        # the chi^2 weights 2 (I - d)/sigma^2 from this I: no second forward pass
-        iv_t, iv_s = prob._inv_var(inv_var)
-        gw = 2.0 * (model - prob._rows(data, (prob.n_rays,), "data"))
-        gw = gw * iv_t if iv_t is not None else gw * iv_s
-        g_table = prob.table_vjp(None, gw, blocks=blocks)
+        c2, g, g_table = self.problem.chi2_spectral_grad(None, data, inv_var,
+                                                         blocks=(self._host, self._dev))
         d_wn = self._c * (self._bnu @ g_table)
         if len(self.names):
             d_wn = d_wn + self._K.t() @ g[self._eff_index]
