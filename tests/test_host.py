@@ -170,6 +170,56 @@ def test_adjoint_workspace_size():
     assert ws(blk.ctypes.data, n, 9, None) == -1
 
 
+def _call_table(lib, block, spectral, mode=0, grad=0x2000, ws_bytes=1 << 20, n_rays=4):
+    dummy = ct.c_void_p(0x1000)  # never dereferenced: every case fails host validation
+    fn = lib.zodi_spectral_vjp if spectral else lib.zodi_table_vjp
+    return fn(block.ctypes.data, dummy, dummy, n_rays, dummy, 1, dummy, dummy, 0, dummy, mode,
+              dummy, dummy, None, 1.0, None, ct.c_void_p(grad) if grad else None, dummy,
+              ws_bytes, None, None)
+
+
+@cpu_only
+def test_table_and_spectral_vjp_workspace_and_validation(cloud_ring_model, planck18_style):
+    """Workspace: block partials (rows of the table's 100 entries and chi^2, plus the
+    parameter vector for the spectral gradient), then I; the launch keeps as many
+    blocks per CU as the kernel's waves (without a device the library assumes 256
+    CUs): four for the Kelsall registry layouts, three for the RRM ones and for
+    the spectral sums on several groups or with scattering, two for the kind switch."""
+    lib = _lib.load()
+    n = 1_000_000
+    cases = [  # (block, table-only blocks per CU, spectral blocks per CU)
+        (_packed(), 4, 3),  # DIRBE 25 um: F != 0, so the spectral pass sums scattering
+        (_packed(planck18_style, 857 * u.GHz), 4, 3),  # DIRBE layout, no scattering
+        (_packed("planck18", 857 * u.GHz), 4, 4),  # one group, no scattering
+        (_packed("rrm-experimental"), 3, 3),
+        (_packed(cloud_ring_model), 2, 2),
+    ]
+    for pk, per_cu_table, per_cu_spec in cases:
+        blk = np.ascontiguousarray(pk.block)
+        np_ = int(blk["n_params"][0])
+        assert lib.zodi_table_vjp_workspace(blk.ctypes.data, n, None) == (
+            per_cu_table * 256 * 101 + n) * 8
+        assert lib.zodi_spectral_vjp_workspace(blk.ctypes.data, n, None) == (
+            per_cu_spec * 256 * (101 + np_) + n) * 8
+        # a small launch: one block of four waves
+        assert lib.zodi_spectral_vjp_workspace(blk.ctypes.data, 64, None) == (
+            (101 + np_) + 64) * 8
+    blk = np.ascontiguousarray(_packed().block)
+    for spectral in (False, True):
+        assert _call_table(lib, blk, spectral, mode=7) == 1
+        assert b"mode" in lib.zodi_last_error()
+        assert _call_table(lib, blk, spectral, grad=0) == 1
+        assert _call_table(lib, blk, spectral, ws_bytes=8) == 1
+        assert b"workspace" in lib.zodi_last_error()
+    # the spectral gradient writes the parameter rows: their layout must be valid
+    corrupt = blk.copy()
+    corrupt["grad_spec_a"][0, 2] = 300
+    assert lib.zodi_spectral_vjp_workspace(corrupt.ctypes.data, n, None) == -1
+    assert _call_table(lib, corrupt, True) == 2
+    assert b"grad_spec" in lib.zodi_last_error()
+    assert lib.zodi_table_vjp_workspace(corrupt.ctypes.data, n, None) > 0
+
+
 @cpu_only
 def test_spectral_setup_host_validation_errors():
     from zodipy_amd.spectral import SPEC_ALBEDO, SPEC_CALIBRATION, SPEC_EMISSIVITY, SPEC_SOLAR
