@@ -187,6 +187,8 @@ enum {
 typedef struct zodi_launch {
   int32_t block;          /* threads per block: 64, 128, 256, 512 */
   int32_t lanes_per_ray;  /* 1, 2, 4, 8: Gauss-Legendre nodes split over this many lanes */
+                          /* (zodi_forward adds the node terms in node order, so its */
+                          /* results do not depend on it; 0: chosen from n_rays)     */
   int32_t blocks_per_cu;  /* > 0: grid-stride launch of this many blocks per CU; 0 = one block per tile */
   int32_t variant;        /* 0 generic, 1 specialised layout if available */
 } zodi_launch;
@@ -211,6 +213,8 @@ const char* zodi_last_error(void);         /* thread-local message of the last f
  *            end is float64 epsilon (zodipy/line_of_sight.py:72-73: the test covers every
  *            observer of one reference call)
  * out        see ZODI_OUT_*
+ * launch     NULL or a zodi_launch; every launch shape gives the same bits for a
+ *            given variant (each ray's node terms are added in node order)
  */
 zodi_status zodi_forward(const zodi_model* model_host, const zodi_model* model_dev,
                          const double* u, int64_t n_rays,

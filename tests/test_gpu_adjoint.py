@@ -516,14 +516,19 @@ def test_custom_kelsall_layout_adjoint(gpu, labels, lam):
 def test_custom_kelsall_layout_matches_central_fd(gpu):
     """Central differences per component; T_0 and delta take a 1e-7 step (B(T) is
     piecewise linear between the table's temperatures, see
-    test_rrm_vjp_matches_central_fd)."""
+    test_rrm_vjp_matches_central_fd). The emission is linear in the emissivities,
+    albedos and solar irradiance (the scattering branch is on at both ends of an
+    albedo's step), so their central difference is exact for any step: a 1e-3
+    step keeps rounding out of it (an albedo of 0 takes a step of 1e-6)."""
     _, names, theta, v, prob = _custom_setup(_custom_kelsall(("cloud", "ring", "feature")),
                                              25.0)
     w = torch.as_tensor(np.random.default_rng(35).normal(size=v.shape[1]), device="cuda")
     g = prob.vjp(theta, w).cpu().numpy()
     fd = np.empty_like(theta)
+    linear = ("emissivity.", "albedo.", "solar_irradiance")
     for i in range(theta.size):
-        rel = 1e-7 if names[i] in ("T_0", "delta") else 1e-6
+        rel = 1e-7 if names[i] in ("T_0", "delta") else (
+            1e-3 if names[i].startswith(linear) else 1e-6)
         h = rel * max(abs(theta[i]), 1e-3)
         tp, tm = theta.copy(), theta.copy()
         tp[i] += h

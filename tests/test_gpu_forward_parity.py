@@ -554,3 +554,44 @@ def test_registry_variant_with_unshared_band_cutoff(gpu, lanes):
     others = [c for c in range(got.shape[0]) if c != band3]
     assert np.array_equal(got[others], before[others])
     assert not np.array_equal(got[band3], before[band3])
+
+
+LANE_CASES = [("dirbe", 25.0, 50), ("dirbe", 1.25, 50), ("dirbe", 12.0, 7), ("planck18", None, 50),
+              ("rrm-experimental", 25.0, 50), ("cloud_ring", 1.25, 50)]
+
+
+@pytest.mark.parametrize("name,lam,degree", LANE_CASES,
+                         ids=["dirbe25", "dirbe1.25_scatter", "dirbe12_7nodes", "planck18_857",
+                              "rrm25", "cloud_ring_generic"])
+def test_lane_splits_give_identical_bits(gpu, name, lam, degree, cloud_ring_model):
+    """Every lane split and block size of one kernel variant gives the same bits: the
+    L lanes of a ray evaluate L nodes at once and every lane adds their terms in node
+    order (50 nodes leave a partial last step for 4 and 8 lanes, 7 nodes one shorter
+    than 8 lanes). The default launch, which splits small launches, equals one lane per
+    ray; so do per-ray observers."""
+    x = 857 * u.GHz if lam is None else lam * u.micron
+    model = Model(x, name=cloud_ring_model if name == "cloud_ring" else name,
+                  gauss_quad_degree=degree)
+    dm = model.device_model
+    rng = np.random.default_rng(44)
+    counts = rng.integers(1, 300, 5)
+    seg = np.concatenate([[0], np.cumsum(counts)]).astype(np.int64)
+    v = random_dirs(int(seg[-1]), seed=45)
+    ph = rng.uniform(0, 2 * np.pi, counts.size)
+    obs = np.stack([np.cos(ph), np.sin(ph), 0.05 * np.ones_like(ph)])
+    for variant in (1, 0):
+        ref = dm.forward(v, obs, obs, seg_offsets=seg, launch=Launch(256, 1, 0, variant))
+        for lanes in (2, 4, 8):
+            for block in (64, 256):
+                got = dm.forward(v, obs, obs, seg_offsets=seg,
+                                 launch=Launch(block, lanes, 0, variant))
+                assert torch.equal(got, ref), (variant, lanes, block)
+        if variant == 1:
+            assert torch.equal(dm.forward(v, obs, obs, seg_offsets=seg), ref)
+            assert_parity(ref.cpu().numpy(),
+                          evaluate_segments(OracleModel(x, model=model._ipd_model,
+                                                        gauss_quad_degree=degree),
+                                            v, seg, obs, obs))
+    per_ray = np.repeat(obs, counts, axis=1)
+    one = dm.forward(v, per_ray, per_ray, per_ray=True, launch=Launch(256, 1, 0, 1))
+    assert torch.equal(dm.forward(v, per_ray, per_ray, per_ray=True), one)

@@ -155,22 +155,45 @@ __device__ __forceinline__ Node node_geometry(const Ray& y, double h, double mid
   return nd;
 }
 
-// Node contribution of one component to its accumulator.
+// Node contribution of one component to its accumulator, as two factors that
+// node_add adds with one fused multiply-add, node by node in order.
 // Kelsall (zodipy/brightness.py:21-56): (1 - a) eps B n, plus a F/R^2 Phi n with
 // scattering; without scattering the constant (1 - a) eps is applied once to the
 // node sum (finish). RRM (zodipy/brightness.py:59-83): B n, calibration applied
 // in finish.
+struct NodeTerm {
+  double a, b;
+};
+
 template <int INTEGRAND>
-__device__ __forceinline__ double node_term(ParamPtr P, uint32_t flags, const Node& nd,
-                                            double wk, double wB, double dens, double solar) {
+__device__ __forceinline__ NodeTerm node_term(ParamPtr P, uint32_t flags, const Node& nd,
+                                              double wk, double wB, double dens, double solar) {
   if constexpr (INTEGRAND == ZODI_INTEGRAND_KELSALL) {
     if (flags & ZODI_CF_SCATTER) {
       double em = P[ZODI_P_ONE_MINUS_ALBEDO] * (P[ZODI_P_EMISSIVITY] * nd.B);
       em += (P[ZODI_P_ALBEDO] * (solar * (nd.inv_R * nd.inv_R))) * nd.phi;
-      return wk * (em * dens);
+      return {wk, em * dens};
     }
   }
-  return wB * dens;
+  return {wB, dens};
+}
+
+__device__ __forceinline__ double node_add(double acc, NodeTerm t) { return fma(t.a, t.b, acc); }
+
+// The terms of L consecutive nodes, one per lane of each L-lane segment, added to
+// every lane's sums in node order (the first `left` of them; `left` is the same
+// on all lanes).
+template <int L, int C>
+__device__ __forceinline__ void add_in_node_order(double (&acc)[C], const NodeTerm (&t)[C],
+                                                  int left) {
+#pragma unroll
+  for (int s = 0; s < L; ++s) {
+    if (s < left) {
+#pragma unroll
+      for (int j = 0; j < C; ++j)
+        acc[j] = node_add(acc[j], NodeTerm{__shfl(t[j].a, s, L), __shfl(t[j].b, s, L)});
+    }
+  }
 }
 
 // Integrated value of one component from its node sum; h = (stop - start)/2.
@@ -357,37 +380,60 @@ __device__ __forceinline__ void group_pass(Grp<Ks...>, std::integer_sequence<int
     ((scatter = scatter || (M->comp_flags[first + J] & ZODI_CF_SCATTER)), ...);
   }
 
+  // A NaN temperature comes from a NaN position. The Kelsall densities are NaN
+  // there too, so their groups skip the lookup's NaN pass-through; the interstellar
+  // density is a constant and the other RRM kinds mask NaN radii to 0 (as the
+  // reference's boolean masks do), so their groups keep it.
+  constexpr bool kNanFix = ((kinds[J] >= ZODI_FAN && kinds[J] <= ZODI_BROAD_BAND) || ...);
   double acc[C] = {};
-  for (int k = cx.sub; k < cx.K; k += L) {
-    const ModelPtr Mk = opaque(M);
-    const double xk = Mk->gl_x[k];
-    const double wk = Mk->gl_w[k];
-    // A NaN temperature comes from a NaN position. The Kelsall densities are NaN
-    // there too, so their groups skip the lookup's NaN pass-through; the interstellar
-    // density is a constant and the other RRM kinds mask NaN radii to 0 (as the
-    // reference's boolean masks do), so their groups keep it.
-    constexpr bool kNanFix = ((kinds[J] >= ZODI_FAN && kinds[J] <= ZODI_BROAD_BAND) || ...);
-    Node nd = node_geometry<kNanFix>(y, h, mid, xk, us, uo, mdelta, cx.tb);
-    if constexpr (INTEGRAND == ZODI_INTEGRAND_KELSALL) {
-      if (scatter) {
-        nd.phi = phase_function(h * xk + mid, y.ux, y.uy, y.uz, nd.X0, nd.X1, nd.X2, nd.R,
-                                Mk->shared);
+  if constexpr (L == 1) {
+    for (int k = 0; k < cx.K; ++k) {
+      const ModelPtr Mk = opaque(M);
+      const double xk = Mk->gl_x[k];
+      const double wk = Mk->gl_w[k];
+      Node nd = node_geometry<kNanFix>(y, h, mid, xk, us, uo, mdelta, cx.tb);
+      if constexpr (INTEGRAND == ZODI_INTEGRAND_KELSALL) {
+        if (scatter) {
+          nd.phi = phase_function(h * xk + mid, y.ux, y.uy, y.uz, nd.X0, nd.X1, nd.X2, nd.R,
+                                  Mk->shared);
+        }
       }
+      const double wB = wk * nd.B;
+      double cut = 0.0;  // a band's cut-off, for the bands that share it (in order)
+      ((acc[J] = node_add(
+            acc[J], node_term<INTEGRAND>(
+                        Mk->comp[first + J], Mk->comp_flags[first + J], nd, wk, wB,
+                        density_k<kinds[J], (PROPS >> (4 * J)) & 15, cut_mode<PROPS, J, Ks...>()>(
+                            Mk->comp[first + J], Mk->comp_flags[first + J], nd, fdir[J], cut),
+                        Mk->shared[ZODI_S_SOLAR]))),
+       ...);
     }
-    const double wB = wk * nd.B;
-    double cut = 0.0;  // a band's cut-off, for the bands that share it (in order)
-    ((acc[J] += node_term<INTEGRAND>(
-          Mk->comp[first + J], Mk->comp_flags[first + J], nd, wk, wB,
-          density_k<kinds[J], (PROPS >> (4 * J)) & 15, cut_mode<PROPS, J, Ks...>()>(
-              Mk->comp[first + J], Mk->comp_flags[first + J], nd, fdir[J], cut),
-          Mk->shared[ZODI_S_SOLAR])),
-     ...);
-  }
-  if constexpr (L > 1) {
-#pragma unroll
-    for (int j = 0; j < C; ++j) {
-#pragma unroll
-      for (int off = L / 2; off > 0; off >>= 1) acc[j] += __shfl_xor(acc[j], off, L);
+  } else {
+    // L lanes evaluate L consecutive nodes at once; every lane then adds the L
+    // terms in node order, so each lane's sums are those of one lane per ray, bit
+    // for bit. Past the last node a lane repeats it, and its term is not added.
+    for (int k0 = 0; k0 < cx.K; k0 += L) {
+      const int k = k0 + cx.sub < cx.K ? k0 + cx.sub : cx.K - 1;
+      const ModelPtr Mk = opaque(M);
+      const double xk = Mk->gl_x[k];
+      const double wk = Mk->gl_w[k];
+      Node nd = node_geometry<kNanFix>(y, h, mid, xk, us, uo, mdelta, cx.tb);
+      if constexpr (INTEGRAND == ZODI_INTEGRAND_KELSALL) {
+        if (scatter) {
+          nd.phi = phase_function(h * xk + mid, y.ux, y.uy, y.uz, nd.X0, nd.X1, nd.X2, nd.R,
+                                  Mk->shared);
+        }
+      }
+      const double wB = wk * nd.B;
+      double cut = 0.0;
+      NodeTerm t[C];
+      ((t[J] = node_term<INTEGRAND>(
+            Mk->comp[first + J], Mk->comp_flags[first + J], nd, wk, wB,
+            density_k<kinds[J], (PROPS >> (4 * J)) & 15, cut_mode<PROPS, J, Ks...>()>(
+                Mk->comp[first + J], Mk->comp_flags[first + J], nd, fdir[J], cut),
+            Mk->shared[ZODI_S_SOLAR])),
+       ...);
+      add_in_node_order<L>(acc, t, cx.K - k0);
     }
   }
   const bool write = y.active && cx.sub == 0;

@@ -106,10 +106,11 @@ zodi_status validate_model(const zodi_model* m) {
 
 
 // Minimum waves per SIMD the register allocator must leave room for, per layout
-// (1 = no constraint). Measured on MI355X: DIRBE and Kelsall4 run at 5 (89 and
-// 80 VGPRs in their registry variants, no spill; DIRBE 2.7 % faster than at 4,
+// (1 = no constraint). Measured on MI355X: DIRBE and Kelsall4 run at 5 (93 and
+// 84 VGPRs in their registry variants, no spill; DIRBE 2.7 % faster than at 4,
 // and 6 waves with a spill gains nothing); RRM at 4 (128 VGPRs with a small
-// spill), 15 % faster than its unconstrained 190 VGPRs / 2 waves.
+// spill), 15 % faster than its unconstrained 190 VGPRs / 2 waves. The kind
+// switch (forward_generic, 169-175 VGPRs) runs at 2.
 template <class LAYOUT>
 struct ForwardOccupancy {
   static constexpr int value = 1;
@@ -180,7 +181,11 @@ __device__ __forceinline__ void generic_groups(const Ctx& cx, const FwdArgs& a, 
       sAcc[j][tid] = 0.0;
       scatter = scatter || (M->comp_flags[first + j] & ZODI_CF_SCATTER);
     }
-    for (int k = cx.sub; k < cx.K; k += L) {
+    // L lanes evaluate L consecutive nodes; each lane adds the L terms in node
+    // order (as forward_layout does), past the last node repeating it unadded.
+    for (int k0 = 0; k0 < cx.K; k0 += L) {
+      const int k = k0 + cx.sub < cx.K ? k0 + cx.sub : cx.K - 1;
+      const int left = cx.K - k0;
       const double xk = M->gl_x[k];
       const double wk = M->gl_w[k];
       Node nd = node_geometry(y, h, mid, xk, us, uo, mdelta, cx.tb);
@@ -198,12 +203,19 @@ __device__ __forceinline__ void generic_groups(const Ctx& cx, const FwdArgs& a, 
         const FeatureDir fd = is_feature(kind) ? feature_dir(P, y.ex, y.ey) : FeatureDir{1.0, 0.0};
         const double dens =
             density(kind, P, flags, nd.X0, nd.X1, nd.X2, nd.R, nd.inv_R, nd.R2, fd);
-        sAcc[j][tid] += node_term<INTEGRAND>(P, flags, nd, wk, wB, dens, cx.solar);
+        const NodeTerm t = node_term<INTEGRAND>(P, flags, nd, wk, wB, dens, cx.solar);
+        double acc = sAcc[j][tid];
+        if (L == 1) {
+          acc = node_add(acc, t);
+        } else {
+          for (int s = 0; s < L && s < left; ++s)
+            acc = node_add(acc, NodeTerm{__shfl(t.a, s, L), __shfl(t.b, s, L)});
+        }
+        sAcc[j][tid] = acc;
       }
     }
     for (int j = 0; j < cnt; ++j) {
-      double acc = sAcc[j][tid];
-      for (int off = L / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, L);
+      const double acc = sAcc[j][tid];
       const double e = finish<INTEGRAND>(M->comp[first + j], M->comp_flags[first + j], acc, h,
                                          cx.calibration);
       if (a.out_mode == ZODI_OUT_COMPS) {
@@ -378,6 +390,38 @@ FwdKernel pick_generic(int block, int L) {
   }
 }
 
+constexpr int kGenericWaves = 2;
+
+// Resident waves per SIMD of the kernel pick_forward chooses.
+int forward_waves(const zodi_model* m, int variant) {
+  switch (variant == 0 ? kLayoutGeneric : match_layout(m)) {
+    case kLayoutDirbe:
+      return ForwardOccupancy<LayoutDirbe>::value;
+    case kLayoutKelsall4:
+      return ForwardOccupancy<LayoutKelsall4>::value;
+    case kLayoutRRM:
+      return ForwardOccupancy<LayoutRRM>::value;
+    case kLayoutRRMSplit:
+      return ForwardOccupancy<LayoutRRMSplit>::value;
+    default:
+      return kGenericWaves;
+  }
+}
+
+// Lanes per ray when the caller leaves it open: the node terms are added in node
+// order whatever the split, so it only sets the speed. A small launch whose one
+// lane per ray leaves SIMDs idle, or one wave on a SIMD with nothing to hide its
+// latency, splits each ray's nodes over up to 8 lanes while the waves still fit
+// the resident slots (kernel waves per SIMD x 4 SIMDs x CUs). Measured (DIRBE,
+// one observer): 3072 rays 102 -> 36 us at 8 lanes, 49 152 rays 107 -> 76 us at
+// 4, and 196 608 rays fastest at 1.
+int default_lanes(const zodi_model* m, int variant, int64_t n_rays) {
+  const int64_t slots = (int64_t)forward_waves(m, variant) * 4 * cu_count();
+  for (int L : {8, 4, 2})
+    if ((n_rays * L + 63) / 64 <= slots) return L;
+  return 1;
+}
+
 FwdKernel pick_forward(const zodi_model* m, int block, int L, int variant) {
   const LayoutId id = variant == 0 ? kLayoutGeneric : match_layout(m);
   switch (id) {
@@ -454,14 +498,16 @@ zodi_status zodi_forward(const zodi_model* model_host, const zodi_model* model_d
     return fail(ZODI_ERR_ARG, "unknown out_mode %d", out_mode);
   if (n_rays == 0) return ZODI_OK;
 
-  int block = 256, L = 1, per_cu = 0, variant = 1;
+  int block = 256, L = 0, per_cu = 0, variant = 1;
   if (launch != nullptr) {
     if (launch->block) block = launch->block;
-    if (launch->lanes_per_ray) L = launch->lanes_per_ray;
+    L = launch->lanes_per_ray;
     per_cu = launch->blocks_per_cu;
     variant = launch->variant;
   }
   if (variant < 0 || variant > 1) return fail(ZODI_ERR_ARG, "unknown variant %d", variant);
+  if (L < 0) return fail(ZODI_ERR_ARG, "lanes_per_ray %d < 0", L);
+  if (L == 0) L = default_lanes(model_host, variant, n_rays);
   if (L > block) return fail(ZODI_ERR_ARG, "lanes_per_ray %d > block %d", L, block);
   FwdKernel k = pick_forward(model_host, block, L, variant);
   if (k == nullptr)
