@@ -307,6 +307,30 @@ def test_beyond_bits_follow_cutoffs():
     assert np.all(allb == (bits[0] | bits[1] | bits[2]))
 
 
+
+def test_beyond_bits_few_observers_match_array_path():
+    """A few observers take a plain-Python path; it gives the array path's masks,
+    at the cutoff radii themselves, for NaN, infinite and zero distances, with and
+    without the OR over observers, and through PackedModel.beyond_bits."""
+    from zodipy_amd.packing import beyond_bits_for
+
+    pk = _packed("rrm-experimental")
+    radii = np.concatenate([pk.group_r_in, pk.group_r_out, [np.nan, np.inf, 0.0, 1.0]])
+    rng = np.random.default_rng(5)
+    for _ in range(300):
+        n = int(rng.integers(1, 9))
+        r = np.where(rng.uniform(size=n) < 0.5, rng.choice(radii, size=n),
+                     rng.uniform(0.0, 6.0, size=n))
+        many = np.concatenate([r, np.full(20, 3.0)])  # past the few-observer size
+        want = beyond_bits_for(many, pk.group_r_in, pk.group_r_out, False)[:n]
+        got = beyond_bits_for(r, pk.group_r_in, pk.group_r_out, False)
+        assert got.dtype == np.uint32 and np.array_equal(got, want)
+        got_any = beyond_bits_for(r, pk.group_r_in, pk.group_r_out, True)
+        assert got_any.shape == (1,) and got_any[0] == np.bitwise_or.reduce(want)
+    obs = rng.normal(size=(3, 5))
+    many = np.concatenate([obs, rng.normal(size=(3, 20))], axis=1)
+    assert np.array_equal(pk.beyond_bits(obs, False), pk.beyond_bits(many, False)[:5])
+
 def test_param_layout_dirbe():
     lay = param_layout(model_registry.get_model("dirbe"))
     assert lay.n_params == 78

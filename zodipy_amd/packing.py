@@ -22,6 +22,8 @@ for the observer distance.
 
 from __future__ import annotations
 
+import math
+
 from dataclasses import dataclass
 
 import numpy as np
@@ -265,9 +267,19 @@ class PackedModel:
         whole set (one reference ``evaluate`` call covers every observer in it).
         """
         obs = np.asarray(obs_xyz, dtype=np.float64).reshape(3, -1)
-        x_0, y_0, z_0 = obs
-        r_obs = np.sqrt(x_0**2 + y_0**2 + z_0**2)
+        if obs.shape[1] <= _FEW_OBSERVERS:
+            # the same operations per observer in plain floats (x * x is x**2 exactly,
+            # and both square roots are correctly rounded)
+            r_obs = np.array([math.sqrt(x * x + y * y + z * z) for x, y, z in obs.T.tolist()])
+        else:
+            x_0, y_0, z_0 = obs
+            r_obs = np.sqrt(x_0**2 + y_0**2 + z_0**2)
         return beyond_bits_for(r_obs, self.group_r_in, self.group_r_out, any_over_all)
+
+
+# Up to this many observers the cutoff masks are formed in plain Python: a single
+# call's array passes cost more than the arithmetic.
+_FEW_OBSERVERS = 8
 
 
 def beyond_bits_for(r_obs: np.ndarray, group_r_in, group_r_out, any_over_all: bool) -> np.ndarray:
@@ -277,6 +289,23 @@ def beyond_bits_for(r_obs: np.ndarray, group_r_in, group_r_out, any_over_all: bo
     neither); with ``any_over_all`` the OR over all observers.
     """
     r_obs = np.asarray(r_obs, dtype=np.float64)
+    if r_obs.size <= _FEW_OBSERVERS:
+        radii = list(zip(np.asarray(group_r_in, dtype=np.float64).tolist(),
+                         np.asarray(group_r_out, dtype=np.float64).tolist()))
+        vals = []
+        for r in r_obs.reshape(-1).tolist():
+            b = 0
+            shift = 0
+            for a, c in radii:
+                b |= ((r > a) << shift) | ((r > c) << (shift + 1))
+                shift += 2
+            vals.append(b)
+        if any_over_all:
+            acc = 0
+            for b in vals:
+                acc |= b
+            return np.array([acc], dtype=np.uint32)
+        return np.array(vals, dtype=np.uint32).reshape(r_obs.shape)
     bits = np.zeros(r_obs.shape, dtype=np.uint32)
     for g in range(len(group_r_in)):
         bits |= (r_obs > group_r_in[g]).astype(np.uint32) << np.uint32(2 * g)

@@ -94,9 +94,16 @@ class DeviceModel:
 
     def prepare_observers(self, obs_xyz, earth_xyz, seg_offsets=None, per_ray: bool = False):
         """Upload observer data; returns (seg, n_seg, obs, earth, beyond, mode)."""
+        return self._prepare(obs_xyz, earth_xyz, seg_offsets, per_ray)[0]
+
+    def _prepare(self, obs_xyz, earth_xyz, seg_offsets=None, per_ray: bool = False,
+                 rows: np.ndarray | None = None):
+        """``prepare_observers``, plus host ``rows`` (float64, C order) uploaded in the
+        same copy when all of it is small: ((seg, n_seg, obs, earth, beyond, mode),
+        rows on the device or None)."""
         if per_ray and isinstance(obs_xyz, torch.Tensor) and obs_xyz.is_cuda and \
                 isinstance(earth_xyz, torch.Tensor) and earth_xyz.is_cuda:
-            return self._device_per_ray(obs_xyz, earth_xyz)
+            return self._device_per_ray(obs_xyz, earth_xyz), None
         obs_np = np.asarray(obs_xyz.cpu() if isinstance(obs_xyz, torch.Tensor) else obs_xyz,
                             dtype=np.float64).reshape(3, -1)
         earth_np = np.asarray(
@@ -130,7 +137,12 @@ class DeviceModel:
                  np.ascontiguousarray(beyond.view(np.int32))]
         if mode == OBS_PER_SEGMENT:
             parts.insert(0, np.ascontiguousarray(seg_np))
-        if sum(a.nbytes for a in parts) <= 1 << 20:
+        small = sum(a.nbytes for a in parts) <= 1 << 20
+        with_rows = rows is not None and small and sum(a.nbytes for a in parts) + rows.nbytes \
+            <= 1 << 20
+        if with_rows:
+            parts.append(rows)
+        if small:
             # One upload for all of them (a small call's fixed cost is mostly its
             # copies): the arrays at 8-byte offsets of one buffer, viewed in place.
             sizes = [a.nbytes for a in parts]
@@ -144,9 +156,10 @@ class DeviceModel:
         else:
             # np.array copies: broadcast views (per-ray observers) are read-only
             up = [torch.from_numpy(np.array(a)).to(self.device) for a in parts]
+        rows_dev = up.pop() if with_rows else None
         seg = up.pop(0) if mode == OBS_PER_SEGMENT else None
         obs, earth, beyond_t = up
-        return seg, n_seg, obs, earth, beyond_t, mode
+        return (seg, n_seg, obs, earth, beyond_t, mode), rows_dev
 
     def forward_raw(self, u: torch.Tensor, seg, n_seg: int, obs: torch.Tensor,
                     earth: torch.Tensor, beyond: torch.Tensor, mode: int, out: torch.Tensor,
@@ -177,9 +190,22 @@ class DeviceModel:
         test over all of them); otherwise (3, n_seg) with ``seg_offsets``;
         a single (3,) observer is one segment.
         """
-        u = _f64_rows(u, 3, "unit vectors", self.device)
-        n_rays = u.shape[1]
+        u_host = None
+        if not isinstance(u, torch.Tensor):
+            # host pointings: checked here, uploaded with the observer data when small
+            u_host = np.ascontiguousarray(np.asarray(u, dtype=np.float64))
+            if u_host.shape == (3,):
+                u_host = u_host.reshape(3, 1)
+            if u_host.ndim != 2 or u_host.shape[0] != 3:
+                msg = f"unit vectors must have shape (3, n), got {u_host.shape}"
+                raise ValueError(msg)
+            n_rays = u_host.shape[1]
+        else:
+            u = _f64_rows(u, 3, "unit vectors", self.device)
+            n_rays = u.shape[1]
         if per_ray and isinstance(obs_xyz, torch.Tensor) and obs_xyz.is_cuda:
+            if u_host is not None:
+                u = _f64_rows(u_host, 3, "unit vectors", self.device)
             if obs_xyz.reshape(3, -1).shape[1] != n_rays:
                 msg = f"per-ray observers must be (3, {n_rays}), got {tuple(obs_xyz.shape)}"
                 raise ValueError(msg)
@@ -199,8 +225,10 @@ class DeviceModel:
         if per_ray and obs_np.shape[1] != n_rays:
             msg = f"per-ray observers must be (3, {n_rays}), got {obs_np.shape}"
             raise ValueError(msg)
-        seg, n_seg, obs, earth, beyond, mode = self.prepare_observers(
-            obs_np, earth_xyz, seg_offsets, per_ray)
+        (seg, n_seg, obs, earth, beyond, mode), u_dev = self._prepare(
+            obs_np, earth_xyz, seg_offsets, per_ray, rows=u_host)
+        if u_host is not None:
+            u = u_dev if u_dev is not None else _f64_rows(u_host, 3, "unit vectors", self.device)
         if mode == OBS_PER_SEGMENT and int(seg_offsets[-1]) != n_rays:
             msg = f"seg_offsets[-1] = {int(seg_offsets[-1])} but there are {n_rays} rays"
             raise ValueError(msg)
