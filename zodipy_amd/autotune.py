@@ -6,9 +6,10 @@ block whenever one retires, which is dynamic scheduling at tile granularity
 already, so a persistent launch could only save the per-block prologue (staging
 about 9 KB of tables into LDS, ~0.1 % of a block's work) and never shortens the
 tail (one tile). Static grid-stride launches (blocks_per_cu > 0) measured 10 %
-slower, because equal tile counts per block are not equal work. Each candidate
-is timed with HIP events on a representative slice of the real batch and the
-fastest is kept. Results are cached as JSON keyed by device name, model layout
+slower, because equal tile counts per block are not equal work. Every candidate
+gives the same bits (lanes sharing a ray add its node terms in node order), so
+the choice only sets the speed. Each candidate is timed with HIP events on a
+representative slice of the real batch and the fastest is kept. Results are cached as JSON keyed by device name, model layout
 (integrand, component kinds, groups), node count and a power-of-two size
 bucket, in ``$ZODI_TUNE_CACHE`` or ``~/.cache/zodipy_amd/tuning.json``.
 """
