@@ -32,8 +32,10 @@ vector may give its components different temperature laws) -- run kernels
 specialised for them; any other combination of component kinds runs one
 generic kernel that takes a group's components one per pass over its nodes.
 ``SpectralTableFit`` fits the spectral tables themselves on the device, and
-``BandpassFit`` the bandpass weights (with ``LineOfSightProblem.table_vjp``, the
-gradient with respect to the Planck table).
+``BandpassFit`` the bandpass weights (with ``LineOfSightProblem.chi2_spectral_grad``:
+the gradient with respect to the effective spectral parameters and the Planck
+table from one pass over the nodes, ``zodi_spectral_vjp``; ``table_vjp`` gives
+the Planck-table part alone).
 """
 
 from __future__ import annotations
