@@ -111,7 +111,9 @@ zodi_status validate_model(const zodi_model* m) {
 // and 6 waves with a spill gains nothing); RRM at 4 (128 VGPRs with a small
 // spill), 15 % faster than its unconstrained 190 VGPRs / 2 waves and 3.7 %
 // faster than 3 waves without the spill. The kind switch (forward_generic,
-// 169-175 VGPRs) runs at 2.
+// 169-175 VGPRs) runs at 2. The split-lane variants (L > 1) run only where the
+// launch is small, so they ask for 2 and keep their registers (DIRBE: 98 VGPRs
+// instead of 96 with a 16-byte spill; 3072 rays at 8 lanes 36.2 -> 34.6 us).
 template <class LAYOUT>
 struct ForwardOccupancy {
   static constexpr int value = 1;
@@ -140,8 +142,8 @@ struct ForwardOccupancy<LayoutKelsall4> {
 // (the registry parameters; a fit that moves them runs SPEC alone).
 template <class LAYOUT, int BLOCK, int L, bool SPEC, bool SHARE = false>
 __global__ __launch_bounds__(BLOCK)
-__attribute__((amdgpu_waves_per_eu(ForwardOccupancy<LAYOUT>::value))) void forward_layout(
-    FwdArgs a) {
+__attribute__((amdgpu_waves_per_eu(L == 1 ? ForwardOccupancy<LAYOUT>::value : 2))) void
+forward_layout(FwdArgs a) {
   __shared__ double sTab[4 * ZODI_N_TEMPS];
   const ModelPtr M = as_const(a.m);
   stage_math_tables();
