@@ -549,8 +549,8 @@ using AdjKernel = void (*)(AdjArgs);
 AdjKernel generic_adjoint_kernel(int integrand, int split, bool jac);
 
 // Resident waves per SIMD the table kernels are compiled for (their register
-// budgets): the Kelsall registry layouts; the RRM ones, and the Kelsall ones with
-// the spectral sums of scattering components; the kind switch.
+// budgets): the registry layouts; the Kelsall ones with the spectral sums of
+// scattering components; the kind switch.
 constexpr int kTableWavesLayout = 4;
 constexpr int kTableWavesWide = 3;
 constexpr int kTableWavesGeneric = 2;

@@ -409,20 +409,16 @@ __device__ __forceinline__ void run_table(Layout<INTEGRAND, Gs...> lay, const Ct
     eff_ray_finish<INTEGRAND>(cx.M, sh, row, lane);
 }
 
-template <int INTEGRAND, class... Gs>
-constexpr int layout_groups(Layout<INTEGRAND, Gs...>) {
-  return sizeof...(Gs);
-}
-
 // Resident waves per SIMD of the registry-layout kernels (their register budget):
-// the RRM layouts, and the Kelsall ones with the spectral sums where those meet
-// the scattering sums or the ray state later groups keep live (geometry, Earth
-// position), need more registers than four waves leave.
+// four, or three for the Kelsall layouts' spectral sums with the scattering terms
+// (at four those spill more and ran 20 % slower). Measured on MI355X, four waves
+// with a small spill beat three without it for the RRM layouts (5 %) and for the
+// DIRBE spectral sums without scattering (3.5 %).
 template <class LAYOUT, bool EFF, bool NOSCAT>
 constexpr int table_waves() {
-  constexpr bool kWide = layout_integrand<LAYOUT>() == ZODI_INTEGRAND_RRM ||
-                         (EFF && (!NOSCAT || layout_groups(LAYOUT{}) > 1));
-  return kWide ? kTableWavesWide : kTableWavesLayout;
+  return layout_integrand<LAYOUT>() == ZODI_INTEGRAND_KELSALL && EFF && !NOSCAT
+             ? kTableWavesWide
+             : kTableWavesLayout;
 }
 
 // SPEC: the registry forms (RegistryProps, with the shared band cut-off when the
