@@ -267,3 +267,20 @@ def test_spectral_vjp_empty_and_launches(gpu):
     none = torch.zeros(0, dtype=torch.float64, device="cuda")
     c2, g, t = empty.chi2_spectral_grad(theta, none)
     assert float(c2) == 0.0 and torch.all(g == 0) and torch.all(t == 0)
+
+
+def test_spectral_chi2_per_ray_inverse_variance(gpu):
+    """chi^2 mode with a per-ray 1/sigma^2: the spectral entries and chi^2 equal
+    zodi_adjoint's Jacobian path, the table part zodi_table_vjp's."""
+    prob = _problem(Model(1.25 * u.micron))
+    theta = prob.space.theta0
+    data, iv = _data(prob, theta, 14)
+    rng = np.random.default_rng(15)
+    ivr = torch.as_tensor(iv * rng.uniform(0.5, 2.0, prob.n_rays), device="cuda")
+    c2, g_th, g_tab = prob.chi2_spectral_grad(theta, data, ivr)
+    c2_j, g_j = prob.chi2_and_grad(theta, data, ivr)
+    assert float(c2) == pytest.approx(float(c2_j), rel=1e-12)
+    _assert_spectral_entries(g_th, g_j, _spectral_mask(prob), "per-ray inverse variance")
+    c2_t, gt = prob.chi2_table_grad(theta, data, ivr)
+    assert float(c2) == float(c2_t)
+    torch.testing.assert_close(g_tab, gt, rtol=1e-12, atol=1e-12 * float(gt.abs().max()))
