@@ -792,6 +792,8 @@ def main() -> int:
             "launch": {"block": launch.block, "lanes_per_ray": launch.lanes_per_ray,
                        "blocks_per_cu": launch.blocks_per_cu},
         },
+        "component_samples_per_s": value * dm.n_comps,
+        "n_comps": dm.n_comps,
         "roofline": roofline,
         "cpu_baseline": cpu,
         "cpu_baselines": cpu_sky_base,
