@@ -284,3 +284,35 @@ def test_spectral_chi2_per_ray_inverse_variance(gpu):
     c2_t, gt = prob.chi2_table_grad(theta, data, ivr)
     assert float(c2) == float(c2_t)
     torch.testing.assert_close(g_tab, gt, rtol=1e-12, atol=1e-12 * float(gt.abs().max()))
+
+
+
+def test_bandpass_fit_on_the_dirbe_25um_band(gpu):
+    """A fit of the DIRBE 25 um band's own 82 weights (the reference's test data):
+    the gradient against central differences on eight weights (1e-6), orthogonal
+    to w."""
+    import json
+    from pathlib import Path
+
+    d = json.loads((Path(__file__).parent / "golden" / "dirbe_25um_bandpass.json").read_text())
+    x = np.asarray(d["wavelength_micron"]) * u.micron
+    w0 = np.asarray(d["weight"], dtype=np.float64)
+    prob = _problem(Model(x, weights=w0))
+    fit = BandpassFit(prob)
+    data, iv = _data(prob, prob.space.theta0, 16)
+    w = w0 * (1.0 + 0.1 * np.random.default_rng(17).uniform(-1, 1, w0.size))
+    c2, g = fit.chi2_and_grad(w, data, iv)
+    g = g.cpu().numpy()
+    assert float(c2) == pytest.approx(_chi2(prob, (fit._host, fit._dev), data, iv), rel=1e-12)
+    assert abs(float(np.dot(g, w))) <= 1e-10 * float(np.abs(g * w).sum())
+    pick = np.argsort(np.abs(g * w))[-8:]
+    for i in pick:
+        h = 1e-6 * w[i]
+        vals = []
+        for sgn in (1.0, -1.0):
+            wp = w.copy()
+            wp[i] += sgn * h
+            fit.set_weights(wp)
+            vals.append(_chi2(prob, (fit._host, fit._dev), data, iv))
+        fd = (vals[0] - vals[1]) / (2 * h)
+        assert g[i] == pytest.approx(fd, rel=1e-6), i

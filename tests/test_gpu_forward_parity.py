@@ -91,6 +91,24 @@ def test_bandpass_gaussian_25um(gpu):
     assert_parity(got, want)
 
 
+
+def test_bandpass_dirbe_25um_instrument(gpu):
+    """The DIRBE 25 um band's own bandpass (82 wavelengths and weights from the
+    reference's test data, tests/golden/dirbe_25um_bandpass.json), on an
+    off-ecliptic observer."""
+    import json
+    from pathlib import Path
+
+    d = json.loads((Path(__file__).parent / "golden" / "dirbe_25um_bandpass.json").read_text())
+    x = np.asarray(d["wavelength_micron"]) * u.micron
+    w = np.asarray(d["weight"])
+    v = random_dirs(700, seed=17)
+    obs = np.array([0.7, 0.7, 0.05])
+    got = Model(x, weights=w).evaluate_vectors(v, obspos=obs, earthpos=obs,
+                                               return_comps=True).value
+    want = OracleModel(x, weights=w).evaluate_vectors(v, obs, obs)
+    assert_parity(got, want)
+
 def test_bandpass_ghz_on_micron_model(gpu):
     """Bandpass in GHz on the micron-tabulated DIRBE model (unit-axis quirk kept)."""
     x = np.linspace(11000, 13000, 20) * u.GHz

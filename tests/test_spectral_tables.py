@@ -34,8 +34,19 @@ def gaussian_bandpass_25um():
     return x, np.exp(-0.5 * ((x.value - 25.0) / 2.5) ** 2)
 
 
+def dirbe_25um_instrument_bandpass():
+    """The DIRBE 25 um band's 82 wavelengths and weights (the reference's test data)."""
+    import json
+    from pathlib import Path
+
+    d = json.loads((Path(__file__).parent / "golden" / "dirbe_25um_bandpass.json").read_text())
+    return np.asarray(d["wavelength_micron"]) * u.micron, np.asarray(d["weight"])
+
+
 CASES = {
     "dirbe_25um": dict(x=25 * u.micron),
+    "dirbe_25um_instrument": dict(x=dirbe_25um_instrument_bandpass()[0],
+                                  weights=dirbe_25um_instrument_bandpass()[1]),
     "dirbe_18.5um": dict(x=18.5 * u.micron),
     "dirbe_bandpass_um": dict(x=gaussian_bandpass_25um()[0], weights=gaussian_bandpass_25um()[1]),
     "dirbe_bandpass_ghz": dict(x=np.linspace(11000, 13000, 20) * u.GHz, weights=np.ones(20)),
