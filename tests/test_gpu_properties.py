@@ -102,3 +102,29 @@ def test_evaluate_properties_and_values(gpu, freq, frame, n, tod, comps, data):
         again = model.evaluate(coords, obspos=obs * u.AU, earthpos=earth * u.AU,
                                return_comps=comps, nprocesses=4)
         assert np.array_equal(again.value, out.value)
+
+
+def test_output_and_input_shapes(gpu):
+    """zodipy's tests/test_evaluate.py:135-169: output shapes with and without
+    components, and the valid obstime/obspos/ncoords combinations (Earth positions
+    given explicitly)."""
+    m = Model(x=20 * u.micron)
+    ncomps = len(model_registry.get_model("dirbe").comps)
+    t0 = Time("2021-01-01T00:00:00")
+    earth1 = np.array([0.2, 0.97, 0.0]) * u.AU
+    one = SkyCoord(20, 30, unit=u.deg, obstime=t0)
+    assert m.evaluate(one, return_comps=True, earthpos=earth1).shape == (ncomps, 1)
+    assert m.evaluate(one, return_comps=False, earthpos=earth1).shape == (1,)
+    six = SkyCoord([20, 22, 24, 26, 28, 30], [30, 28, 26, 24, 22, 20], unit=u.deg, obstime=t0)
+    assert m.evaluate(six, return_comps=True, earthpos=earth1).shape == (ncomps, 6)
+    assert m.evaluate(six, return_comps=False, earthpos=earth1).shape == (6,)
+    obstimes = Time([float(t0.mjd) + i for i in range(4)], format="mjd")
+    ph = 0.2 + 0.0172 * np.arange(4)
+    obsposes = np.stack([np.cos(ph), np.sin(ph), np.zeros(4)]) * u.AU
+    four = SkyCoord([20, 30, 40, 50], [30, 40, 30, 20], unit=u.deg, obstime=obstimes)
+    assert m.evaluate(SkyCoord([20, 30], [30, 40], unit=u.deg, obstime=t0),
+                      earthpos=earth1).shape == (2,)
+    a = m.evaluate(four, earthpos=obsposes)
+    b = m.evaluate(four, obspos=obsposes, earthpos=obsposes)
+    assert a.shape == b.shape == (4,)
+    assert np.array_equal(a.value, b.value)  # obspos "earth" is Earth's position

@@ -383,6 +383,62 @@ def test_x_input():
         Model(np.linspace(20, 25, 10) * u.micron)
 
 
+
+def test_invalid_sky_coord():
+    """zodipy's tests/test_evaluate.py:98-116: the same exception types, raised
+    before any position is looked up or any GPU work."""
+    from zodipy_amd.coords import SkyCoord, Time
+
+    m = Model(x=20 * u.micron)
+    with pytest.raises(TypeError):
+        m.evaluate(20)
+    with pytest.raises(ValueError):
+        m.evaluate(SkyCoord(20, 30, unit=u.deg))
+    with pytest.raises(ValueError):
+        m.evaluate(SkyCoord(20, 30, unit=u.deg,
+                            obstime=Time(["2021-01-01T00:00:00", "2021-01-01T00:02:00"])))
+
+
+def test_evaluate_invalid_obspos():
+    """zodipy's tests/test_evaluate.py:119-132."""
+    from zodipy_amd.coords import SkyCoord, Time
+
+    m = Model(x=20 * u.micron)
+    sc = SkyCoord(20, 30, unit=u.deg, obstime=Time("2021-01-01T00:00:00"))
+    earth = np.array([1.0, 0.0, 0.0]) * u.AU
+    with pytest.raises(ValueError):
+        m.evaluate(sc, obspos="invalid", earthpos=earth)
+    with pytest.raises(TypeError):
+        m.evaluate(sc, obspos=[30, 40, 50])
+    with pytest.raises(u.UnitConversionError):
+        m.evaluate(sc, obspos=[30, 40, 50] * u.s)
+    with pytest.raises(ValueError):
+        m.evaluate(sc, obspos=[[30, 40, 50], [30, 40, 50]] * u.AU)
+
+
+def test_evaluate_input_shape_errors():
+    """The mismatched obstime/obspos/ncoords cases of zodipy's tests/test_evaluate.py:151-195."""
+    from zodipy_amd.coords import SkyCoord, Time
+
+    m = Model(x=20 * u.micron)
+    t0 = Time("2021-01-01T00:00:00")
+    obstimes = Time([float(t0.mjd) + i for i in range(4)], format="mjd")
+    obsposes = np.array([[1.0, 0.1, 0.0], [0.9, 0.3, 0.0], [0.8, 0.5, 0.0],
+                         [0.7, 0.7, 0.0]]).T * u.AU  # (3, 4)
+    four = SkyCoord([20, 30, 40, 50], [30, 40, 30, 20], unit=u.deg, obstime=obstimes)
+    one = SkyCoord(20, 30, unit=u.deg, obstime=t0)
+    earth1, earth4 = np.array([1.0, 0.0, 0.0]) * u.AU, obsposes
+    with pytest.raises(ValueError, match="obstime"):  # obstime > sky_coord
+        m.evaluate(SkyCoord([20, 30], [30, 40], unit=u.deg, obstime=obstimes), earthpos=earth4)
+    with pytest.raises(ValueError, match="obstime"):  # obspos > sky_coord / obstime
+        m.evaluate(one, obspos=obsposes, earthpos=earth1)
+    with pytest.raises(ValueError, match="obstime"):  # obstimes > obspos
+        m.evaluate(four, obspos=[1, 2, 3] * u.AU, earthpos=earth4)
+    with pytest.raises(ValueError, match="shape"):
+        m.evaluate(four, obspos=[[1, -0.4, 0.2]] * u.AU, earthpos=earth4)
+    with pytest.raises(ValueError, match="shape"):
+        m.evaluate(four, obspos=obsposes[:2], earthpos=earth4)  # (2, 4)
+
 def test_weights_input():
     bandpass = np.random.default_rng(0).random(10)
     with pytest.raises(ValueError):

@@ -136,6 +136,16 @@ class Model:
             if not isinstance(obspos, units.Quantity):
                 msg = "The observer position is not a string or a Quantity."
                 raise TypeError(msg)
+            try:  # the unit before any position is looked up, as the reference fails
+                obspos.to_value(units.AU)
+            except units.UnitConversionError as error:
+                msg = "The observer position must be in length units."
+                raise units.UnitConversionError(msg) from error
+            if obspos.ndim > 2 or (obspos.ndim == 2 and obspos.shape[0] != 3) or (
+                obspos.ndim == 1 and obspos.shape[0] != 3
+            ):
+                msg = "The observer position must have shape (3,) or (3, ncoords)."
+                raise ValueError(msg)
             if (obspos.ndim > 1 and skycoord.obstime.size != obspos.shape[-1]) or (
                 obspos.ndim == 1 and skycoord.obstime.size != 1
             ):
