@@ -110,8 +110,8 @@ zodi_status validate_model(const zodi_model* m) {
 // 84 VGPRs in their registry variants, no spill; DIRBE 2.7 % faster than at 4,
 // and 6 waves with a spill gains nothing); RRM at 4 (128 VGPRs with a small
 // spill), 15 % faster than its unconstrained 190 VGPRs / 2 waves and 3.7 %
-// faster than 3 waves without the spill. The kind switch (forward_generic,
-// 169-175 VGPRs) runs at 2. The split-lane variants (L > 1) run only where the
+// faster than 3 waves without the spill. The kind switch (forward_generic)
+// runs at 3. The split-lane variants (L > 1) run only where the
 // launch is small, so they ask for 2 and keep their registers (DIRBE: 98 VGPRs
 // instead of 96 with a 16-byte spill; 3072 rays at 8 lanes 36.2 -> 34.6 us).
 template <class LAYOUT>
@@ -230,8 +230,12 @@ __device__ __forceinline__ void generic_groups(const Ctx& cx, const FwdArgs& a, 
   }
 }
 
+// Three waves per SIMD (167-168 VGPRs, up to a 16-byte spill with split lanes):
+// 22-30 % faster than the unconstrained 169-175 VGPRs at two (DIRBE through the
+// kind switch on 196 608 to 3.1 M rays; the same on 49 152).
 template <int BLOCK, int L>
-__global__ __launch_bounds__(BLOCK) void forward_generic(FwdArgs a) {
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(3))) void
+forward_generic(FwdArgs a) {
   __shared__ double sTab[4 * ZODI_N_TEMPS];
   __shared__ double sAcc[kMaxSlots][512];
   const ModelPtr M = as_const(a.m);
@@ -393,7 +397,7 @@ FwdKernel pick_generic(int block, int L) {
   }
 }
 
-constexpr int kGenericWaves = 2;
+constexpr int kGenericWaves = 3;
 
 // Resident waves per SIMD of the kernel pick_forward chooses.
 int forward_waves(const zodi_model* m, int variant) {
