@@ -184,7 +184,7 @@ def test_table_and_spectral_vjp_workspace_and_validation(cloud_ring_model, planc
     parameter vector for the spectral gradient), then I; the launch keeps as many
     blocks per CU as the kernel's waves (without a device the library assumes 256
     CUs): four for the registry layouts, three for the Kelsall layouts' spectral
-    sums with scattering, two for the kind switch."""
+    sums with scattering and for the kind switch."""
     lib = _lib.load()
     n = 1_000_000
     cases = [  # (block, table-only blocks per CU, spectral blocks per CU)
@@ -192,7 +192,7 @@ def test_table_and_spectral_vjp_workspace_and_validation(cloud_ring_model, planc
         (_packed(planck18_style, 857 * u.GHz), 4, 4),  # DIRBE layout, no scattering
         (_packed("planck18", 857 * u.GHz), 4, 4),  # one group, no scattering
         (_packed("rrm-experimental"), 4, 4),
-        (_packed(cloud_ring_model), 2, 2),
+        (_packed(cloud_ring_model), 3, 3),
     ]
     for pk, per_cu_table, per_cu_spec in cases:
         blk = np.ascontiguousarray(pk.block)
