@@ -91,7 +91,6 @@ def test_bandpass_gaussian_25um(gpu):
     assert_parity(got, want)
 
 
-
 def test_bandpass_dirbe_25um_instrument(gpu):
     """The DIRBE 25 um band's own bandpass (82 wavelengths and weights from the
     reference's test data, tests/golden/dirbe_25um_bandpass.json), on an
@@ -108,6 +107,7 @@ def test_bandpass_dirbe_25um_instrument(gpu):
                                                return_comps=True).value
     want = OracleModel(x, weights=w).evaluate_vectors(v, obs, obs)
     assert_parity(got, want)
+
 
 def test_bandpass_ghz_on_micron_model(gpu):
     """Bandpass in GHz on the micron-tabulated DIRBE model (unit-axis quirk kept)."""
@@ -599,8 +599,8 @@ def test_lane_splits_give_identical_bits(gpu, name, lam, degree, cloud_ring_mode
     obs = np.stack([np.cos(ph), np.sin(ph), 0.05 * np.ones_like(ph)])
     for variant in (1, 0):
         ref = dm.forward(v, obs, obs, seg_offsets=seg, launch=Launch(256, 1, 0, variant))
-        for lanes in (2, 4, 8):
-            for block in (64, 256):
+        for lanes in (1, 2, 4, 8):
+            for block in (64, 256) if lanes > 1 else (64, 128, 512):
                 got = dm.forward(v, obs, obs, seg_offsets=seg,
                                  launch=Launch(block, lanes, 0, variant))
                 assert torch.equal(got, ref), (variant, lanes, block)

@@ -164,9 +164,9 @@ forward_layout(FwdArgs a) {
 }
 
 // ------------------------------------------------------------ generic kernel
-template <int INTEGRAND>
+template <int INTEGRAND, int BLOCK>
 __device__ __forceinline__ void generic_groups(const Ctx& cx, const FwdArgs& a, const Ray& y,
-                                               double (*sAcc)[512], int tid, int L,
+                                               double (*sAcc)[BLOCK], int tid, int L,
                                                double& total) {
   const ModelPtr M = cx.M;
   const int n_groups = M->n_groups;
@@ -237,7 +237,7 @@ template <int BLOCK, int L>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(3))) void
 forward_generic(FwdArgs a) {
   __shared__ double sTab[4 * ZODI_N_TEMPS];
-  __shared__ double sAcc[kMaxSlots][512];
+  __shared__ double sAcc[kMaxSlots][BLOCK];  // one node sum per lane and component slot
   const ModelPtr M = as_const(a.m);
   stage_math_tables();
   load_table(M, sTab);
