@@ -160,15 +160,16 @@ __device__ __forceinline__ Node node_geometry(const Ray& y, double h, double mid
 // Kelsall (zodipy/brightness.py:21-56): (1 - a) eps B n, plus a F/R^2 Phi n with
 // scattering; without scattering the constant (1 - a) eps is applied once to the
 // node sum (finish). RRM (zodipy/brightness.py:59-83): B n, calibration applied
-// in finish.
+// in finish. NOSCAT: no component of the block scatters (no ZODI_CF_SCATTER flag,
+// which the host checks), so the flag is not tested.
 struct NodeTerm {
   double a, b;
 };
 
-template <int INTEGRAND>
+template <int INTEGRAND, bool NOSCAT = false>
 __device__ __forceinline__ NodeTerm node_term(ParamPtr P, uint32_t flags, const Node& nd,
                                               double wk, double wB, double dens, double solar) {
-  if constexpr (INTEGRAND == ZODI_INTEGRAND_KELSALL) {
+  if constexpr (INTEGRAND == ZODI_INTEGRAND_KELSALL && !NOSCAT) {
     if (flags & ZODI_CF_SCATTER) {
       double em = P[ZODI_P_ONE_MINUS_ALBEDO] * (P[ZODI_P_EMISSIVITY] * nd.B);
       em += (P[ZODI_P_ALBEDO] * (solar * (nd.inv_R * nd.inv_R))) * nd.phi;
@@ -197,11 +198,11 @@ __device__ __forceinline__ void add_in_node_order(double (&acc)[C], const NodeTe
 }
 
 // Integrated value of one component from its node sum; h = (stop - start)/2.
-template <int INTEGRAND>
+template <int INTEGRAND, bool NOSCAT = false>
 __device__ __forceinline__ double finish(ParamPtr P, uint32_t flags, double acc, double h,
                                          double calibration) {
   if constexpr (INTEGRAND == ZODI_INTEGRAND_KELSALL) {
-    if (flags & ZODI_CF_SCATTER) return __dmul_rn(acc, h);
+    if (!NOSCAT && (flags & ZODI_CF_SCATTER)) return __dmul_rn(acc, h);
     return __dmul_rn(__dmul_rn(acc, P[ZODI_P_SCALE]), h);
   } else {
     return __dmul_rn(__dmul_rn(acc, calibration), h);
@@ -356,7 +357,7 @@ constexpr int cut_mode() {
 }
 
 // PROPS: 4 bits per component of the group (see density_k), component J at bits 4J.
-template <int INTEGRAND, int L, uint32_t PROPS, int... Ks, int... J>
+template <int INTEGRAND, int L, uint32_t PROPS, bool NOSCAT, int... Ks, int... J>
 __device__ __forceinline__ void group_pass(Grp<Ks...>, std::integer_sequence<int, J...>,
                                            const Ctx& cx, const FwdArgs& a, const Ray& y, int g,
                                            int first, double& total) {
@@ -376,7 +377,7 @@ __device__ __forceinline__ void group_pass(Grp<Ks...>, std::integer_sequence<int
                                    : FeatureDir{1.0, 0.0}),
    ...);
   bool scatter = false;
-  if constexpr (INTEGRAND == ZODI_INTEGRAND_KELSALL) {
+  if constexpr (INTEGRAND == ZODI_INTEGRAND_KELSALL && !NOSCAT) {
     ((scatter = scatter || (M->comp_flags[first + J] & ZODI_CF_SCATTER)), ...);
   }
 
@@ -401,7 +402,7 @@ __device__ __forceinline__ void group_pass(Grp<Ks...>, std::integer_sequence<int
       const double wB = wk * nd.B;
       double cut = 0.0;  // a band's cut-off, for the bands that share it (in order)
       ((acc[J] = node_add(
-            acc[J], node_term<INTEGRAND>(
+            acc[J], node_term<INTEGRAND, NOSCAT>(
                         Mk->comp[first + J], Mk->comp_flags[first + J], nd, wk, wB,
                         density_k<kinds[J], (PROPS >> (4 * J)) & 15, cut_mode<PROPS, J, Ks...>()>(
                             Mk->comp[first + J], Mk->comp_flags[first + J], nd, fdir[J], cut),
@@ -427,7 +428,7 @@ __device__ __forceinline__ void group_pass(Grp<Ks...>, std::integer_sequence<int
       const double wB = wk * nd.B;
       double cut = 0.0;
       NodeTerm t[C];
-      ((t[J] = node_term<INTEGRAND>(
+      ((t[J] = node_term<INTEGRAND, NOSCAT>(
             Mk->comp[first + J], Mk->comp_flags[first + J], nd, wk, wB,
             density_k<kinds[J], (PROPS >> (4 * J)) & 15, cut_mode<PROPS, J, Ks...>()>(
                 Mk->comp[first + J], Mk->comp_flags[first + J], nd, fdir[J], cut),
@@ -439,8 +440,9 @@ __device__ __forceinline__ void group_pass(Grp<Ks...>, std::integer_sequence<int
   const bool write = y.active && cx.sub == 0;
   (
       [&] {
-        const double e = finish<INTEGRAND>(M->comp[first + J], M->comp_flags[first + J],
-                                           acc[J], h, cx.calibration);
+        const double e = finish<INTEGRAND, NOSCAT>(M->comp[first + J],
+                                                   M->comp_flags[first + J], acc[J], h,
+                                                   cx.calibration);
         if (a.out_mode == ZODI_OUT_COMPS) {
           if (write) a.out[(int64_t)(first + J) * a.n_rays + y.r] = e;
         } else {
@@ -476,21 +478,21 @@ constexpr int layout_integrand() {
 }
 
 // PROPS: 4 bits per component of the layout (see density_k), component c at bits 4c.
-template <int L, uint32_t PROPS, int INTEGRAND, class... Gs, int... G>
+template <int L, uint32_t PROPS, bool NOSCAT, int INTEGRAND, class... Gs, int... G>
 __device__ __forceinline__ void run_layout_groups(Layout<INTEGRAND, Gs...>,
                                                   std::integer_sequence<int, G...>, const Ctx& cx,
                                                   const FwdArgs& a, const Ray& y, double& total) {
-  (group_pass<INTEGRAND, L, props_shift(PROPS, group_first<Gs...>(G))>(
+  (group_pass<INTEGRAND, L, props_shift(PROPS, group_first<Gs...>(G)), NOSCAT>(
        Gs{}, std::make_integer_sequence<int, Gs::size>{}, cx, a, y, G, group_first<Gs...>(G),
        total),
    ...);
 }
 
-template <int L, uint32_t PROPS, int INTEGRAND, class... Gs>
+template <int L, uint32_t PROPS, bool NOSCAT, int INTEGRAND, class... Gs>
 __device__ __forceinline__ void run_layout(Layout<INTEGRAND, Gs...> lay, const Ctx& cx,
                                            const FwdArgs& a, const Ray& y, double& total) {
-  run_layout_groups<L, PROPS>(lay, std::make_integer_sequence<int, sizeof...(Gs)>{}, cx, a, y,
-                              total);
+  run_layout_groups<L, PROPS, NOSCAT>(lay, std::make_integer_sequence<int, sizeof...(Gs)>{}, cx,
+                                      a, y, total);
 }
 
 // Component properties of a layout kernel's registry variant: every component is

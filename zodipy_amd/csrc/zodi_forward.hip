@@ -107,8 +107,8 @@ zodi_status validate_model(const zodi_model* m) {
 
 // Minimum waves per SIMD the register allocator must leave room for, per layout
 // (1 = no constraint). Measured on MI355X: DIRBE and Kelsall4 run at 5 (93 and
-// 84 VGPRs in their registry variants, no spill; DIRBE 2.7 % faster than at 4,
-// and 6 waves with a spill gains nothing); RRM at 4 (128 VGPRs with a small
+// 84 VGPRs in their registry variants, 89 and 79 without scattering code, no
+// spill; DIRBE 2.7 % faster than at 4, and 6 waves with a spill gains nothing); RRM at 4 (128 VGPRs with a small
 // spill), 15 % faster than its unconstrained 190 VGPRs / 2 waves and 3.7 %
 // faster than 3 waves without the spill. The kind switch (forward_generic)
 // runs at 3. The split-lane variants (L > 1) run only where the
@@ -139,8 +139,11 @@ struct ForwardOccupancy<LayoutKelsall4> {
 // registry models' (RegistryProps); the host runs it only for blocks that meet
 // its preconditions (props_match). SHARE (with SPEC): the bands that share a
 // radial cut-off reuse it (kShareCut), for blocks whose delta_r values allow it
-// (the registry parameters; a fit that moves them runs SPEC alone).
-template <class LAYOUT, int BLOCK, int L, bool SPEC, bool SHARE = false>
+// (the registry parameters; a fit that moves them runs SPEC alone). NOSCAT (with
+// SHARE): no component scatters, so no scattering code is compiled in (no flag
+// test per component and node; 1.5-2 % faster on the DIRBE 12 um, planck18 and
+// six-component 857 GHz batches).
+template <class LAYOUT, int BLOCK, int L, bool SPEC, bool SHARE = false, bool NOSCAT = false>
 __global__ __launch_bounds__(BLOCK)
 __attribute__((amdgpu_waves_per_eu(L == 1 ? ForwardOccupancy<LAYOUT>::value : 2))) void
 forward_layout(FwdArgs a) {
@@ -158,7 +161,7 @@ forward_layout(FwdArgs a) {
         !SPEC ? 0u
               : (SHARE ? RegistryProps<LAYOUT>::value
                        : props_without_share(RegistryProps<LAYOUT>::value));
-    run_layout<L, kProps>(LAYOUT{}, cx, a, y, total);
+    run_layout<L, kProps, NOSCAT>(LAYOUT{}, cx, a, y, total);
     if (a.out_mode == ZODI_OUT_SUM && y.active && cx.sub == 0) a.out[y.r] = total;
   }
 }
@@ -340,9 +343,23 @@ FwdKernel pick_layout_block(int block, int L) {
   }
 }
 
+// Whether no component of the block has its scattering term (ZODI_CF_SCATTER).
+inline bool no_scattering(const zodi_model* m) {
+  for (int c = 0; c < m->n_comps; ++c)
+    if (m->comp_flags[c] & ZODI_CF_SCATTER) return false;
+  return true;
+}
+
+template <class LAYOUT, int BLOCK>
+FwdKernel pick_shared_cut(const zodi_model* m) {
+  if (no_scattering(m)) return forward_layout<LAYOUT, BLOCK, 1, true, true, true>;
+  return forward_layout<LAYOUT, BLOCK, 1, true, true>;
+}
+
 // The registry variant when the block meets its preconditions (with the shared
-// cut-offs for one lane per ray, the launch the tuner picks for large batches),
-// else the one that tests each component's flags.
+// cut-offs for one lane per ray, the launch the tuner picks for large batches,
+// and without scattering code when no component scatters), else the one that
+// tests each component's flags.
 template <class LAYOUT>
 FwdKernel pick_layout(const zodi_model* m, int block, int L) {
   constexpr uint32_t kProps = RegistryProps<LAYOUT>::value;
@@ -350,13 +367,13 @@ FwdKernel pick_layout(const zodi_model* m, int block, int L) {
     if (L == 1 && props_match(m, kProps)) {
       switch (block) {
         case 64:
-          return forward_layout<LAYOUT, 64, 1, true, true>;
+          return pick_shared_cut<LAYOUT, 64>(m);
         case 128:
-          return forward_layout<LAYOUT, 128, 1, true, true>;
+          return pick_shared_cut<LAYOUT, 128>(m);
         case 256:
-          return forward_layout<LAYOUT, 256, 1, true, true>;
+          return pick_shared_cut<LAYOUT, 256>(m);
         case 512:
-          return forward_layout<LAYOUT, 512, 1, true, true>;
+          return pick_shared_cut<LAYOUT, 512>(m);
         default:
           return nullptr;
       }
