@@ -586,7 +586,10 @@ def test_lane_splits_give_identical_bits(gpu, name, lam, degree, cloud_ring_mode
     L lanes of a ray evaluate L nodes at once and every lane adds their terms in node
     order (50 nodes leave a partial last step for 4 and 8 lanes, 7 nodes one shorter
     than 8 lanes). The default launch, which splits small launches, equals one lane per
-    ray; so do per-ray observers."""
+    ray; so do per-ray observers. For the blocks where no component scatters (DIRBE
+    25 um, planck18) the one-lane registry kernel is the one compiled without
+    scattering code and the split-lane kernels keep it, so the same bits pin that
+    variant too."""
     x = 857 * u.GHz if lam is None else lam * u.micron
     model = Model(x, name=cloud_ring_model if name == "cloud_ring" else name,
                   gauss_quad_degree=degree)
