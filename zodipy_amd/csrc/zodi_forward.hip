@@ -108,7 +108,8 @@ zodi_status validate_model(const zodi_model* m) {
 // Minimum waves per SIMD the register allocator must leave room for, per layout
 // (1 = no constraint). Measured on MI355X: DIRBE and Kelsall4 run at 5 (93 and
 // 84 VGPRs in their registry variants, 89 and 79 without scattering code, no
-// spill; DIRBE 2.7 % faster than at 4, and 6 waves with a spill gains nothing); RRM at 4 (128 VGPRs with a small
+// spill; DIRBE 2.7 % faster than at 4, and 6 waves with a spill gains nothing,
+// 0.8 % slower without scattering code); RRM at 4 (128 VGPRs with a small
 // spill), 15 % faster than its unconstrained 190 VGPRs / 2 waves and 3.7 %
 // faster than 3 waves without the spill. The kind switch (forward_generic)
 // runs at 3. The split-lane variants (L > 1) run only where the
