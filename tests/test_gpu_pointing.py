@@ -66,6 +66,17 @@ def test_lonlat_to_vectors(gpu, frame, degrees):
     np.testing.assert_allclose(got.cpu().numpy(), want, rtol=0, atol=ATOL)
 
 
+def test_lonlat_large_and_non_finite_angles(gpu):
+    """Angles beyond 64 radians take the library's sin/cos, non-finite ones give NaN."""
+    lon = np.array([3700.0, -2.0e4, 1.0e7, 123.4, np.nan, np.inf, 30.0])
+    lat = np.array([10.0, -89.0, 45.0, 4000.0, 0.0, 0.0, -np.inf])
+    got = pointing.lonlat_to_vectors(lon, lat, degrees=True).cpu().numpy()
+    with np.errstate(invalid="ignore"):
+        want = coords.lonlat_to_xyz(np.radians(lon), np.radians(lat))
+    np.testing.assert_allclose(got, want, rtol=0, atol=ATOL)
+    assert np.isnan(got[:2, 4:]).all() and np.isnan(got[:, 6]).all()
+
+
 def test_evaluate_skycoord_uses_device_vectors(gpu):
     """Model.evaluate on a galactic SkyCoord equals evaluate_vectors on host vectors."""
     rng = np.random.default_rng(3)

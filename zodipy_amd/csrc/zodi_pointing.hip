@@ -122,11 +122,22 @@ __global__ __launch_bounds__(256) void pix2vec_kernel(int64_t nside, const int64
     } else {
       double sth, phi, s, c;
       ring_pixel(g, p, z, sth, phi);
-      sincos(phi, &s, &c);
+      zodi_trig::sincos_moderate(phi, &s, &c);  // phi in [0, 2 pi)
       x = sth * c;
       y = sth * s;
     }
     store_rotated(R, identity != 0, x, y, z, out, n, i);
+  }
+}
+
+// sin and cos of an angle in radians: the moderate-argument form for |x| <= 64
+// (any longitude or latitude given in [-360, 720] degrees), the library's
+// otherwise (and for non-finite x).
+__device__ __forceinline__ void sincos_angle(double x, double* s, double* c) {
+  if (fabs(x) <= 64.0) {
+    zodi_trig::sincos_moderate(x, s, c);
+  } else {
+    sincos(x, s, c);
   }
 }
 
@@ -139,8 +150,8 @@ __global__ __launch_bounds__(256) void lonlat_kernel(const double* __restrict__ 
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
     double sl, cl, sb, cb;
-    sincos(lon[i] * scale, &sl, &cl);
-    sincos(lat[i] * scale, &sb, &cb);
+    sincos_angle(lon[i] * scale, &sl, &cl);
+    sincos_angle(lat[i] * scale, &sb, &cb);
     store_rotated(R, identity != 0, cb * cl, cb * sl, sb, out, n, i);
   }
 }
